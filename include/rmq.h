@@ -1,0 +1,142 @@
+/*
+ * rmq.h -- C ABI of the MI355X batch range-minimum engine (librtxrmq.so).
+ *
+ * Replaces the reference's RTX_blocks launcher
+ *     template <typename T> T* rtx_rmq(int alg, int n, int bs, int q,
+ *                                      float* darray, int2* dquery, CmdArgs args)
+ * (src/rtx.h:24-177, called from src/main.cu:132-137 for alg 5 with T=float
+ * and alg 105 with T=int).  The reference fuses build and query in one call
+ * and returns a malloc'd host array; here build and query are separate,
+ * stream-ordered and write caller-owned device buffers.
+ *
+ * Semantics (identical to the reference CPU BASE, algs 0/100,
+ * src/cpu_methods.h:7-53):
+ *   value  = min A[l..r], the exact float32 bits of the minimum element;
+ *   index  = the smallest i in [l, r] with A[i] equal to that minimum.
+ * Queries are int32 pairs {l, r}, inclusive, 0 <= l <= r < n, the layout of
+ * the reference's int2 queries (src/rand.cuh:149-150).  Inputs must not
+ * contain NaN (the reference's scan order is undefined for NaN as well).
+ * Invalid queries never fault: they produce value NaN and index -1 and are
+ * counted (rmq_query_ex's n_invalid).
+ *
+ * Errors: every call returns RMQ_OK (0) or a negative RMQ_ERR_* code; no
+ * exception or abort crosses the ABI (the reference's CUDA_CHECK prints and
+ * continues, common/common.h:57-69).
+ */
+#ifndef RTXRMQ_AMD_RMQ_H
+#define RTXRMQ_AMD_RMQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* HIP's stream handle (the same typedef hip_runtime_api.h makes). */
+typedef struct ihipStream_t *hipStream_t;
+
+typedef struct rmq_handle rmq_handle_t;
+
+#define RMQ_OK 0
+#define RMQ_ERR_INVALID_ARG (-1)
+#define RMQ_ERR_OOM (-2)
+#define RMQ_ERR_HIP (-3)
+#define RMQ_ERR_DEVICE (-4)    /* handle used on a device other than the one it was built on */
+#define RMQ_ERR_BAD_QUERY (-5) /* returned by rmq_query_check when a query had l > r or r >= n */
+
+/* Array kinds for the generators (the reference draws U(0,1] with cuRAND,
+ * src/rand.cuh:15-21; these draw exact grid values from a counter-based
+ * stream so that host and device produce the same array). */
+#define RMQ_GEN_UNIFORM 0 /* k * 2^-24, k uniform in [0, 2^24): U[0,1) exactly      */
+#define RMQ_GEN_TIES16 1  /* integers 0..15 as float (heavy ties)                  */
+#define RMQ_GEN_DESC 2    /* bits 0x3F800000 - 1 - i: strictly descending, positive */
+#define RMQ_GEN_CONST 3   /* all 1.0f                                               */
+
+/* Query tuning; zero fields take the defaults (rmq_query_opts_default). */
+typedef struct rmq_query_opts {
+    int32_t short_max;   /* ranges of at most this many elements go to the group scan   */
+    int32_t group_lanes; /* lanes per short query: 4, 8, 16 or 64                       */
+    int32_t long_wgs;    /* persistent workgroups for the long-range kernel (0 = auto)  */
+    int32_t reserved;
+} rmq_query_opts_t;
+
+/* Per-kernel device time of one rmq_query_timed call, milliseconds. */
+typedef struct rmq_query_times {
+    float short_ms; /* binning pass fused with the short-range group scans */
+    float long_ms;  /* long-range wave scans (edge blocks + table)          */
+    float total_ms; /* from the first launch to the last                   */
+    int32_t n_long; /* queries that went to the long-range kernel          */
+} rmq_query_times_t;
+
+/* ---- index build: replaces gen_vertices_blocks_dev + gen_triangles_dev +
+ *      buildASFromDeviceData (src/rtx.h:40-60,96; src/device_tools.cuh:243-287;
+ *      src/rtx_functions.h:289-379) and compute_min_blocks for alg 105
+ *      (src/rtx.h:3-22).  Builds the per-block leftmost min/argmin over blocks
+ *      of bs elements (nb = ceil(n/bs), src/rtx.h:42) and a sparse table over
+ *      them.  d_A must stay alive and unchanged while the handle is used: the
+ *      edge-block scans read it.  n <= 2^31-1, 1 <= bs.  flags is reserved (0).
+ *      Stream-ordered; the handle is usable on stream s at once, on other
+ *      streams after s is synchronised. ---- */
+int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out);
+
+/* ---- batched query: replaces the optixLaunch loop running
+ *      __raygen__rmq_blocks / __raygen__rmq_blocks_idx (src/rtx.h:150-155,
+ *      src/rtx_kernels.cu:85-180,350-449).  d_Q holds q pairs {l, r};
+ *      d_val (alg 5) and/or d_idx (alg 105) receive q results; either may be
+ *      NULL but not both.  Stream-ordered and asynchronous; the handle is
+ *      read-only, so concurrent queries on different streams are safe. ---- */
+int rmq_query(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx, hipStream_t s);
+
+/* rmq_query with tuning options (NULL = defaults). */
+int rmq_query_ex(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                 const rmq_query_opts_t *opts, hipStream_t s);
+
+/* rmq_query_ex that also times its kernels with HIP events recorded on s and
+ * waits for them (a synchronising call, for measurement). */
+int rmq_query_timed(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                    const rmq_query_opts_t *opts, hipStream_t s, rmq_query_times_t *times);
+
+/* Synchronising validation: counts queries with l < 0, l > r or r >= n.
+ * Returns RMQ_OK when none, RMQ_ERR_BAD_QUERY otherwise (count in *n_bad). */
+int rmq_query_check(const int32_t *d_Q, int64_t q, int64_t n, hipStream_t s, int64_t *n_bad);
+
+/* Releases everything the handle owns (not d_A).  NULL is a no-op. */
+void rmq_free(rmq_handle_t *h);
+
+/* Handle geometry: number of blocks, sparse-table levels, device bytes owned. */
+int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int64_t *nb, int32_t *levels,
+                    int64_t *bytes);
+
+/* Default tuning for a handle (what a NULL opts means). */
+int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts);
+
+/* ---- brute force on the device, algs 2/102 and the --check oracle:
+ *      replaces kernel_rmq_basic / kernel_rmq_basic_idx / gpu_rmq_basic
+ *      (src/cuda_methods.cuh:3-169): one thread per query, linear scan from
+ *      A[l] with strict '<'.  Either output may be NULL but not both. ---- */
+int rmq_gpu_base(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                 hipStream_t s);
+
+/* ---- input generators: replace create_random_array_dev (src/rand.cuh:23-43)
+ *      and random_queries_par_cpu (src/rand.cuh:179-201).  Streams depend on
+ *      (seed, index) only, never on the thread count (unlike the reference's
+ *      mt19937(seed*tid), src/rand.cuh:120,140,160).
+ *      lr > 0: fixed length lr; -1 uniform length in [1, n-l]; -2 / -3
+ *      lognormal round(exp(N(ln 2^14 | ln 2^6, 1))) clamped to [1, n-l];
+ *      -4 / -5 lognormal with the reference's scales n/2^8, n/2^15 and
+ *      sigma 0.3 (src/rand.cuh:194-198), clamped; -6 r = n-1. ---- */
+int rmq_gen_array(float *d_A, int64_t n, uint64_t seed, int32_t kind, hipStream_t s);
+int rmq_gen_array_host(float *A, int64_t n, uint64_t seed, int32_t kind);
+int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int32_t nthreads);
+
+/* Static description of a return code. */
+const char *rmq_strerror(int code);
+
+/* Library version string. */
+const char *rmq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTXRMQ_AMD_RMQ_H */

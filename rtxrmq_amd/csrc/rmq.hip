@@ -1,0 +1,832 @@
+// rmq.hip -- CDNA4 (gfx950) kernels and C ABI of the batch range-minimum engine.
+//
+// The reference answers a query with up to three OptiX ray casts into a
+// two-level triangle scene (src/rtx_kernels.cu:85-180): one over the block
+// minima between the edge blocks, one into each edge block.  Here the two
+// levels are plain arrays in HBM:
+//   level 1: the leftmost (min, argmin) of every block of bs elements, with a
+//            sparse table over them (O(1) lookup of any run of whole blocks);
+//   level 0: the array itself, scanned directly for the edge blocks.
+// A query pass is a binning kernel that sends short ranges to group scans
+// (several queries per wave) and queues the rest for a persistent kernel that
+// gives each long range a whole wave: two edge-block scans plus one table
+// lookup, combined so that ties resolve to the leftmost index.
+//
+// All comparisons are compare-and-select on float32 ('<' and '=='), never
+// fminf, so the result is the exact bits of the leftmost minimum element.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "rmq.h"
+
+#define RMQ_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWG = 256;                 // threads per workgroup (4 waves)
+constexpr int kWavesPerWG = kWG / kWave;
+constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
+constexpr int kDefaultShortMax = 1024;
+constexpr int kDefaultGroupLanes = 16;
+
+struct __align__(8) Pair {
+    float v;
+    int32_t i;
+};
+
+// ---------------------------------------------------------------------------
+// per-lane accumulation: lanes visit their elements in increasing index order,
+// so strict '<' keeps the leftmost minimum within a lane.  The sentinel is
+// (+inf, INT_MAX); an element equal to +inf never replaces it, which the
+// final fix-up (value +inf => index l) corrects: with no NaN, a +inf minimum
+// means every element of the range is +inf and the leftmost is l.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void upd(float x, int e, float &bv, int &bi) {
+    const bool t = x < bv;
+    bv = t ? x : bv;
+    bi = t ? e : bi;
+}
+
+__device__ __forceinline__ void upd4(const float4 &x, int e, float &bv, int &bi) {
+    upd(x.x, e, bv, bi);
+    upd(x.y, e + 1, bv, bi);
+    upd(x.z, e + 2, bv, bi);
+    upd(x.w, e + 3, bv, bi);
+}
+
+// (ov, oi) replaces (v, i) when smaller, or equal with a smaller index.
+__device__ __forceinline__ void take_if_better(float ov, int oi, float &v, int &i) {
+    const bool t = (ov < v) || (ov == v && oi < i);
+    v = t ? ov : v;
+    i = t ? oi : i;
+}
+
+// One DPP exchange step.  'old' is the lane's own value, so a lane whose
+// source is out of the row keeps its own pair (idempotent for a minimum).
+template <int CTRL>
+__device__ __forceinline__ void dpp_step(float &v, int &i) {
+    const int ov = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false);
+    take_if_better(__int_as_float(ov), oi, v, i);
+}
+
+// DPP controls: quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2),
+// row_half_mirror (pairs the two quads of each 8), row_mirror (pairs the two
+// halves of each row of 16).  After G steps every lane of an aligned group of
+// G lanes holds the group's minimum.
+template <int G>
+__device__ __forceinline__ void group_reduce(float &v, int &i) {
+    static_assert(G == 4 || G == 8 || G == 16, "DPP group reduction works within a row of 16");
+    dpp_step<0xB1>(v, i);
+    dpp_step<0x4E>(v, i);
+    if constexpr (G >= 8) dpp_step<0x141>(v, i);
+    if constexpr (G >= 16) dpp_step<0x140>(v, i);
+}
+
+// Whole-wave reduction: rows of 16 by DPP, then the four row results are read
+// into scalar registers.  Every lane returns the same pair.
+__device__ __forceinline__ void wave_reduce(float &v, int &i) {
+    group_reduce<16>(v, i);
+    float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const float ov = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), row * 16));
+        const int oi = __builtin_amdgcn_readlane(i, row * 16);
+        take_if_better(ov, oi, rv, ri);
+    }
+    v = rv;
+    i = ri;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Scans A[a..b] (a <= b) with STRIDE cooperating lanes, lane in [0, STRIDE).
+// Head (up to 3 elements before the first 16-byte boundary) and tail (up to
+// 3 after the last) are scalar loads; the aligned middle is float4 loads, U
+// per lane in flight.  Each lane visits head, middle, tail in index order.
+// Every load lies inside [a, b], so nothing past the array end is touched.
+template <int STRIDE, int U>
+__device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
+    const int ah = (a + 3) & ~3;
+    const int bt = (b + 1) & ~3;
+    const int h_end = min(ah, b + 1);
+    const int t_beg = max(bt, h_end);
+    if (lane < h_end - a) upd(A[a + lane], a + lane, bv, bi);
+    const float4 *__restrict__ A4 = reinterpret_cast<const float4 *>(A);
+    int v = (h_end >> 2) + lane;
+    const int v_end = t_beg >> 2;  // exclusive; empty when h_end == t_beg
+    if (h_end < t_beg) {
+        for (; v + (U - 1) * STRIDE < v_end; v += U * STRIDE) {
+            float4 x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = A4[v + u * STRIDE];
+#pragma unroll
+            for (int u = 0; u < U; ++u) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
+        }
+        for (; v < v_end; v += STRIDE) upd4(A4[v], v << 2, bv, bi);
+    }
+    if (lane < b + 1 - t_beg) upd(A[t_beg + lane], t_beg + lane, bv, bi);
+}
+
+// ---------------------------------------------------------------------------
+// build kernels
+// ---------------------------------------------------------------------------
+
+// Leftmost (min, argmin) of every block [b*bs, min(n, (b+1)*bs)).
+// Replaces kernel_min_blocks (src/device_tools.cuh:243-253, one thread per
+// block) and compute_min_blocks (src/rtx.h:3-22, which also overreads the last
+// partial block): here one workgroup per block, float4-coalesced.
+__global__ __launch_bounds__(kWG) void k_block_minloc(const float *__restrict__ A, int n, int bs, int nb,
+                                                      Pair *__restrict__ out) {
+    __shared__ float sv[kWavesPerWG];
+    __shared__ int si[kWavesPerWG];
+    const int tid = threadIdx.x;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int lo = (int)((int64_t)b * bs);
+        const int hi = (int)min((int64_t)n - 1, (int64_t)lo + bs - 1);
+        float bv = INFINITY;
+        int bi = INT_MAX;
+        scan_range<kWG, 2>(A, lo, hi, tid, bv, bi);
+        wave_reduce(bv, bi);
+        if ((tid & (kWave - 1)) == 0) {
+            sv[tid / kWave] = bv;
+            si[tid / kWave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+            for (int w = 1; w < kWavesPerWG; ++w) take_if_better(sv[w], si[w], bv, bi);
+            if (bv == INFINITY) bi = lo;
+            out[b] = Pair{bv, bi};
+        }
+        __syncthreads();
+    }
+}
+
+// Sparse-table level: cur[i] = leftmost min of prev[i], prev[i+half].  The two
+// windows are disjoint and the left one holds the smaller indices, so the
+// left wins ties.
+__global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__ cur, int count, int half) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Pair x = prev[i];
+    const Pair y = prev[i + half];
+    cur[i] = (y.v < x.v) ? y : x;
+}
+
+// ---------------------------------------------------------------------------
+// query kernels
+// ---------------------------------------------------------------------------
+
+struct QueryCounters {
+    uint32_t n_long;
+    uint32_t n_invalid;
+};
+
+__global__ void k_zero_counters(QueryCounters *c) {
+    if (threadIdx.x == 0) {
+        c->n_long = 0;
+        c->n_invalid = 0;
+    }
+}
+
+__device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
+    if (out_v) out_v[qi] = v;
+    if (out_i) out_i[qi] = i;
+}
+
+// Binning pass fused with the short-range scans.  Each wave takes tiles of 64
+// consecutive queries.  Invalid queries get (NaN, -1).  Ranges of more than
+// short_max elements are appended to long_ids (one atomic per wave); the
+// short ones are staged in LDS and answered G lanes per query, 64/G queries
+// per round, every lane taking part in the DPP reduction of each round.
+template <int G>
+__global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A, int n,
+                                                     const int2 *__restrict__ Q, int64_t q, int short_max,
+                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i,
+                                                     int32_t *__restrict__ long_ids, QueryCounters *cnt) {
+    constexpr int QPR = kWave / G;  // queries per round
+    __shared__ int s_l[kWavesPerWG][kWave];
+    __shared__ int s_r[kWavesPerWG][kWave];
+    __shared__ int s_q[kWavesPerWG][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
+    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+        const int64_t qi = t * kWave + lane;
+        const bool in = qi < q;
+        int l = 0, r = -1;
+        if (in) {
+            const int2 p = Q[qi];
+            l = p.x;
+            r = p.y;
+        }
+        const bool ok = in && l >= 0 && l <= r && r < n;
+        if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+        const uint64_t bad = __ballot(in && !ok);
+        if (bad && lane == 0) atomicAdd(&cnt->n_invalid, (uint32_t)__popcll(bad));
+        const bool is_short = ok && (r - l) < short_max;
+        const bool is_long = ok && !is_short;
+        const uint64_t lm = __ballot(is_long);
+        if (lm) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&cnt->n_long, (uint32_t)__popcll(lm));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (is_long) long_ids[base + mask_rank(lm)] = (int32_t)qi;
+        }
+        const uint64_t sm = __ballot(is_short);
+        if (is_short) {
+            const uint32_t k = mask_rank(sm);
+            s_l[wv][k] = l;
+            s_r[wv][k] = r;
+            s_q[wv][k] = (int)(qi - t * kWave);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int n_short = __popcll(sm);
+        const int g = lane / G;
+        const int gl = lane & (G - 1);
+        for (int base = 0; base < n_short; base += QPR) {
+            const int s = base + g;
+            const bool has = s < n_short;
+            float bv = INFINITY;
+            int bi = INT_MAX;
+            int ql = 0;
+            if (has) {
+                ql = s_l[wv][s];
+                const int qr = s_r[wv][s];
+                scan_range<G, 1>(A, ql, qr, gl, bv, bi);
+            }
+            group_reduce<G>(bv, bi);
+            if (has && gl == 0) {
+                if (bv == INFINITY) bi = ql;
+                write_result(out_v, out_i, t * kWave + s_q[wv][s], bv, bi);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Whole-wave short path (group of 64 lanes): one query per round.
+__global__ __launch_bounds__(kWG) void k_query_short64(const float *__restrict__ A, int n,
+                                                       const int2 *__restrict__ Q, int64_t q, int short_max,
+                                                       float *__restrict__ out_v, int32_t *__restrict__ out_i,
+                                                       int32_t *__restrict__ long_ids, QueryCounters *cnt) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
+    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+        const int64_t qi = t * kWave + lane;
+        const bool in = qi < q;
+        int l = 0, r = -1;
+        if (in) {
+            const int2 p = Q[qi];
+            l = p.x;
+            r = p.y;
+        }
+        const bool ok = in && l >= 0 && l <= r && r < n;
+        if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+        const uint64_t bad = __ballot(in && !ok);
+        if (bad && lane == 0) atomicAdd(&cnt->n_invalid, (uint32_t)__popcll(bad));
+        const bool is_short = ok && (r - l) < short_max;
+        const bool is_long = ok && !is_short;
+        const uint64_t lm = __ballot(is_long);
+        if (lm) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&cnt->n_long, (uint32_t)__popcll(lm));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (is_long) long_ids[base + mask_rank(lm)] = (int32_t)qi;
+        }
+        uint64_t sm = __ballot(is_short);
+        while (sm) {
+            const int src = __ffsll((unsigned long long)sm) - 1;
+            sm &= sm - 1;
+            const int ql = __builtin_amdgcn_readlane(l, src);
+            const int qr = __builtin_amdgcn_readlane(r, src);
+            float bv = INFINITY;
+            int bi = INT_MAX;
+            scan_range<kWave, 2>(A, ql, qr, lane, bv, bi);
+            wave_reduce(bv, bi);
+            if (lane == 0) {
+                if (bv == INFINITY) bi = ql;
+                write_result(out_v, out_i, t * kWave + src, bv, bi);
+            }
+        }
+    }
+}
+
+// Long ranges: one wave per query, persistent grid striding over the queue.
+// lB == rB: one in-block scan (src/rtx_kernels.cu:116-135).  Otherwise the
+// left edge block from l, the right edge block up to r (:151-177) and, when
+// whole blocks lie between, the sparse table over block minima (:138-149).
+__global__ __launch_bounds__(kWG) void k_query_long(const float *__restrict__ A, int bs, int nb,
+                                                    const Pair *__restrict__ st, const int2 *__restrict__ Q,
+                                                    const int32_t *__restrict__ long_ids,
+                                                    const QueryCounters *__restrict__ cnt,
+                                                    float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave_global = blockIdx.x * kWavesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wave_stride = gridDim.x * kWavesPerWG;
+    const int n_long = (int)cnt->n_long;
+    for (int k = wave_global; k < n_long; k += wave_stride) {
+        const int qi = long_ids[k];
+        const int2 p = Q[qi];
+        const int l = p.x, r = p.y;
+        const int lB = l / bs, rB = r / bs;
+        float bv = INFINITY;
+        int bi = INT_MAX;
+        float mv = INFINITY;
+        int mi = INT_MAX;
+        if (lB == rB) {
+            scan_range<kWave, kLongUnroll>(A, l, r, lane, bv, bi);
+        } else {
+            if (rB - lB > 1) {
+                const int a = lB + 1, c = rB - lB - 1;
+                const int lev = 31 - __clz(c);
+                const Pair *row = st + (int64_t)lev * nb;
+                const Pair x = row[a];
+                const Pair y = row[a + c - (1 << lev)];
+                const Pair m = (y.v < x.v) ? y : x;
+                mv = m.v;
+                mi = m.i;
+            }
+            scan_range<kWave, kLongUnroll>(A, l, lB * bs + bs - 1, lane, bv, bi);
+            scan_range<kWave, kLongUnroll>(A, rB * bs, r, lane, bv, bi);
+        }
+        wave_reduce(bv, bi);
+        take_if_better(mv, mi, bv, bi);
+        if (lane == 0) {
+            if (bv == INFINITY) bi = l;
+            write_result(out_v, out_i, qi, bv, bi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// brute force on the device (algs 2/102, --check): src/cuda_methods.cuh:3-66
+// ---------------------------------------------------------------------------
+__global__ void k_gpu_base(const float *__restrict__ A, const int2 *__restrict__ Q, int64_t q, int n,
+                           float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= q) return;
+    const int2 p = Q[tid];
+    if (p.x < 0 || p.x > p.y || p.y >= n) {
+        write_result(out_v, out_i, tid, __int_as_float(0x7FC00000), -1);
+        return;
+    }
+    float m = A[p.x];
+    int mi = p.x;
+    for (int i = p.x; i <= p.y; ++i) {
+        const float v = A[i];
+        if (v < m) {
+            m = v;
+            mi = i;
+        }
+    }
+    write_result(out_v, out_i, tid, m, mi);
+}
+
+__global__ void k_count_bad(const int2 *__restrict__ Q, int64_t q, int64_t n, unsigned long long *bad) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool b = false;
+    if (tid < q) {
+        const int2 p = Q[tid];
+        b = p.x < 0 || p.x > p.y || (int64_t)p.y >= n;
+    }
+    const uint64_t m = __ballot(b);
+    if (m && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+// ---------------------------------------------------------------------------
+// generators (same definitions on host and device)
+// ---------------------------------------------------------------------------
+constexpr uint64_t kPhi = 0x9E3779B97F4A7C15ULL;
+
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+__host__ __device__ __forceinline__ uint64_t rng_key(uint64_t seed, uint64_t stream) { return mix64(seed * kPhi + stream); }
+__host__ __device__ __forceinline__ uint64_t rng_at(uint64_t key, uint64_t i) { return mix64(key + (i + 1) * kPhi); }
+
+__host__ __device__ __forceinline__ float gen_value(uint64_t key, int64_t i, int kind) {
+    switch (kind) {
+    case RMQ_GEN_UNIFORM: return (float)(rng_at(key, (uint64_t)i) >> 40) * 0x1p-24f;
+    case RMQ_GEN_TIES16: return (float)(rng_at(key, (uint64_t)i) >> 60);
+    case RMQ_GEN_DESC: {
+        const uint32_t bits = 0x3F800000u - 1u - (uint32_t)i;
+        float f;
+        memcpy(&f, &bits, 4);
+        return f;
+    }
+    default: return 1.0f;
+    }
+}
+
+__global__ void k_gen_array(float *__restrict__ A, int64_t n, uint64_t key, int kind) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) A[i] = gen_value(key, i, kind);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+int g_num_cus = 0;
+std::once_flag g_pool_once;
+
+int device_cus() {
+    if (g_num_cus > 0) return g_num_cus;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    g_num_cus = cus;
+    return cus;
+}
+
+// Keep stream-ordered allocations cached in the default pool between calls.
+void keep_pool_warm() {
+    std::call_once(g_pool_once, [] {
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    });
+}
+
+int hip_rc(hipError_t e) {
+    if (e == hipSuccess) return RMQ_OK;
+    if (e == hipErrorOutOfMemory) return RMQ_ERR_OOM;
+    (void)hipGetLastError();
+    return RMQ_ERR_HIP;
+}
+
+#define RMQ_HIP(call)                         \
+    do {                                      \
+        const int rc_ = hip_rc((call));       \
+        if (rc_ != RMQ_OK) return rc_;        \
+    } while (0)
+
+}  // namespace
+
+struct rmq_handle {
+    const float *A;
+    int64_t n;
+    int32_t bs;
+    int64_t nb;
+    int32_t levels;
+    int device;
+    Pair *st;  // levels * nb pairs; level 0 = block minima
+    int64_t st_bytes;
+};
+
+namespace {
+
+int check_device(const rmq_handle *h) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
+    return dev == h->device ? RMQ_OK : RMQ_ERR_DEVICE;
+}
+
+void fill_default_opts(const rmq_handle *h, rmq_query_opts_t *o) {
+    (void)h;
+    o->short_max = kDefaultShortMax;
+    o->group_lanes = kDefaultGroupLanes;
+    o->long_wgs = 0;
+    o->reserved = 0;
+}
+
+int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts_t *o) {
+    fill_default_opts(h, o);
+    if (in) {
+        if (in->short_max < 0 || in->long_wgs < 0) return RMQ_ERR_INVALID_ARG;
+        if (in->short_max > 0) o->short_max = in->short_max;
+        if (in->group_lanes != 0) {
+            if (in->group_lanes != 4 && in->group_lanes != 8 && in->group_lanes != 16 && in->group_lanes != 64)
+                return RMQ_ERR_INVALID_ARG;
+            o->group_lanes = in->group_lanes;
+        }
+        if (in->long_wgs > 0) o->long_wgs = in->long_wgs;
+    }
+    return RMQ_OK;
+}
+
+int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
+                 int32_t *long_ids, QueryCounters *cnt, hipStream_t s) {
+    const int64_t tiles = (q + kWave - 1) / kWave;
+    const int64_t want = (tiles + kWavesPerWG - 1) / kWavesPerWG;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)device_cus() * 16));
+    const int n = (int)h->n;
+    switch (o.group_lanes) {
+    case 4: k_query_short<4><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
+    case 8: k_query_short<8><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
+    case 16: k_query_short<16><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
+    default: k_query_short64<<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
+    }
+    return hip_rc(hipGetLastError());
+}
+
+int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
+                const int32_t *long_ids, const QueryCounters *cnt, hipStream_t s) {
+    const int64_t want = (q + kWavesPerWG - 1) / kWavesPerWG;
+    int grid = o.long_wgs > 0 ? o.long_wgs : device_cus() * 8;
+    grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, want));
+    k_query_long<<<grid, kWG, 0, s>>>(h->A, h->bs, (int)h->nb, h->st, Q, long_ids, cnt, val, idx);
+    return hip_rc(hipGetLastError());
+}
+
+int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+               const rmq_query_opts_t *opts_in, hipStream_t s, rmq_query_times_t *times) {
+    if (!h || (!d_val && !d_idx) || q < 0 || (q > 0 && !d_Q)) return RMQ_ERR_INVALID_ARG;
+    if (q > INT32_MAX) return RMQ_ERR_INVALID_ARG;
+    int rc = check_device(h);
+    if (rc != RMQ_OK) return rc;
+    rmq_query_opts_t o;
+    rc = resolve_opts(h, opts_in, &o);
+    if (rc != RMQ_OK) return rc;
+    if (q == 0) {
+        if (times) *times = rmq_query_times_t{0.f, 0.f, 0.f, 0};
+        return RMQ_OK;
+    }
+    keep_pool_warm();
+    const size_t ws_bytes = 256 + sizeof(int32_t) * (size_t)q;
+    void *ws = nullptr;
+    RMQ_HIP(hipMallocAsync(&ws, ws_bytes, s));
+    QueryCounters *cnt = reinterpret_cast<QueryCounters *>(ws);
+    int32_t *long_ids = reinterpret_cast<int32_t *>(static_cast<char *>(ws) + 256);
+    const int2 *Q = reinterpret_cast<const int2 *>(d_Q);
+
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (times) {
+        for (auto &e : ev)
+            if ((rc = hip_rc(hipEventCreate(&e))) != RMQ_OK) break;
+    }
+    if (rc == RMQ_OK) {
+        k_zero_counters<<<1, 64, 0, s>>>(cnt);
+        rc = hip_rc(hipGetLastError());
+    }
+    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[0], s));
+    if (rc == RMQ_OK) rc = launch_short(h, Q, q, o, d_val, d_idx, long_ids, cnt, s);
+    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[1], s));
+    if (rc == RMQ_OK) rc = launch_long(h, Q, q, o, d_val, d_idx, long_ids, cnt, s);
+    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[2], s));
+    if (rc == RMQ_OK && times) {
+        QueryCounters hc{0, 0};
+        rc = hip_rc(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+        if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+        if (rc == RMQ_OK) {
+            float a = 0.f, b = 0.f;
+            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+            *times = rmq_query_times_t{a, b, a + b, (int32_t)hc.n_long};
+        }
+    }
+    const int rc_free = hip_rc(hipFreeAsync(ws, s));
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc != RMQ_OK ? rc : rc_free;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out) {
+    if (!out) return RMQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || flags != 0) return RMQ_ERR_INVALID_ARG;
+    const int64_t nb = (n + bs - 1) / bs;
+    int32_t levels = 0;
+    while (((int64_t)1 << levels) <= nb) ++levels;  // floor(log2(nb)) + 1
+
+    rmq_handle *h = new (std::nothrow) rmq_handle{};
+    if (!h) return RMQ_ERR_OOM;
+    h->A = d_A;
+    h->n = n;
+    h->bs = bs;
+    h->nb = nb;
+    h->levels = levels;
+    h->st_bytes = (int64_t)sizeof(Pair) * nb * levels;
+    int rc = hip_rc(hipGetDevice(&h->device));
+    if (rc == RMQ_OK) rc = hip_rc(hipMalloc(&h->st, (size_t)h->st_bytes));
+    if (rc == RMQ_OK) {
+        const int grid = (int)std::min<int64_t>(nb, 65535);
+        k_block_minloc<<<grid, kWG, 0, s>>>(d_A, (int)n, bs, (int)nb, h->st);
+        rc = hip_rc(hipGetLastError());
+    }
+    for (int32_t k = 1; rc == RMQ_OK && k < levels; ++k) {
+        const int half = 1 << (k - 1);
+        const int count = (int)(nb - ((int64_t)1 << k) + 1);
+        k_sparse_level<<<(count + 255) / 256, 256, 0, s>>>(h->st + (int64_t)(k - 1) * nb, h->st + (int64_t)k * nb,
+                                                           count, half);
+        rc = hip_rc(hipGetLastError());
+    }
+    if (rc != RMQ_OK) {
+        if (h->st) (void)hipFree(h->st);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_query(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                      hipStream_t s) {
+    return query_impl(h, d_Q, q, d_val, d_idx, nullptr, s, nullptr);
+}
+
+RMQ_API int rmq_query_ex(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                         const rmq_query_opts_t *opts, hipStream_t s) {
+    return query_impl(h, d_Q, q, d_val, d_idx, opts, s, nullptr);
+}
+
+RMQ_API int rmq_query_timed(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                            const rmq_query_opts_t *opts, hipStream_t s, rmq_query_times_t *times) {
+    if (!times) return RMQ_ERR_INVALID_ARG;
+    return query_impl(h, d_Q, q, d_val, d_idx, opts, s, times);
+}
+
+RMQ_API int rmq_query_check(const int32_t *d_Q, int64_t q, int64_t n, hipStream_t s, int64_t *n_bad) {
+    if (!n_bad || q < 0 || (q > 0 && !d_Q) || n <= 0) return RMQ_ERR_INVALID_ARG;
+    *n_bad = 0;
+    if (q == 0) return RMQ_OK;
+    unsigned long long *d_bad = nullptr;
+    RMQ_HIP(hipMalloc(&d_bad, sizeof(unsigned long long)));
+    unsigned long long hb = 0;
+    int rc = hip_rc(hipMemcpyAsync(d_bad, &hb, sizeof(hb), hipMemcpyHostToDevice, s));
+    if (rc == RMQ_OK) {
+        k_count_bad<<<(unsigned)((q + 255) / 256), 256, 0, s>>>(reinterpret_cast<const int2 *>(d_Q), q, n, d_bad);
+        rc = hip_rc(hipGetLastError());
+    }
+    if (rc == RMQ_OK) rc = hip_rc(hipMemcpyAsync(&hb, d_bad, sizeof(hb), hipMemcpyDeviceToHost, s));
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    (void)hipFree(d_bad);
+    if (rc != RMQ_OK) return rc;
+    *n_bad = (int64_t)hb;
+    return hb ? RMQ_ERR_BAD_QUERY : RMQ_OK;
+}
+
+RMQ_API void rmq_free(rmq_handle_t *h) {
+    if (!h) return;
+    if (h->st) (void)hipFree(h->st);
+    delete h;
+}
+
+RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int64_t *nb, int32_t *levels,
+                            int64_t *bytes) {
+    if (!h) return RMQ_ERR_INVALID_ARG;
+    if (n) *n = h->n;
+    if (bs) *bs = h->bs;
+    if (nb) *nb = h->nb;
+    if (levels) *levels = h->levels;
+    if (bytes) *bytes = h->st_bytes;
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts) {
+    if (!h || !opts) return RMQ_ERR_INVALID_ARG;
+    fill_default_opts(h, opts);
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_gpu_base(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+                         hipStream_t s) {
+    if (!d_A || n <= 0 || n > INT32_MAX || q < 0 || (q > 0 && !d_Q) || (!d_val && !d_idx)) return RMQ_ERR_INVALID_ARG;
+    if (q == 0) return RMQ_OK;
+    const int bsz = 256;  // the reference uses BSIZE=1024 (src/main.cu:14); wave64 prefers 256
+    k_gpu_base<<<(unsigned)((q + bsz - 1) / bsz), bsz, 0, s>>>(d_A, reinterpret_cast<const int2 *>(d_Q), q, (int)n,
+                                                               d_val, d_idx);
+    return hip_rc(hipGetLastError());
+}
+
+RMQ_API int rmq_gen_array(float *d_A, int64_t n, uint64_t seed, int32_t kind, hipStream_t s) {
+    if (!d_A || n <= 0 || kind < 0 || kind > 3) return RMQ_ERR_INVALID_ARG;
+    if (kind == RMQ_GEN_DESC && n > 0x3F7FFFFFLL) return RMQ_ERR_INVALID_ARG;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)device_cus() * 32);
+    k_gen_array<<<grid, 256, 0, s>>>(d_A, n, rng_key(seed, 0), kind);
+    return hip_rc(hipGetLastError());
+}
+
+RMQ_API int rmq_gen_array_host(float *A, int64_t n, uint64_t seed, int32_t kind) {
+    if (!A || n <= 0 || kind < 0 || kind > 3) return RMQ_ERR_INVALID_ARG;
+    if (kind == RMQ_GEN_DESC && n > 0x3F7FFFFFLL) return RMQ_ERR_INVALID_ARG;
+    const uint64_t key = rng_key(seed, 0);
+    for (int64_t i = 0; i < n; ++i) A[i] = gen_value(key, i, kind);
+    return RMQ_OK;
+}
+
+namespace {
+
+// Box-Muller in double; contraction off so every host build rounds alike.
+double lognormal_len(uint64_t k3, uint64_t k4, uint64_t i, double mu, double sigma) {
+#pragma clang fp contract(off)
+    const double u1 = (double)((rng_at(k3, i) >> 11) + 1) * 0x1p-53;  // (0,1]
+    const double u2 = (double)(rng_at(k4, i) >> 11) * 0x1p-53;        // [0,1)
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    const double ang = 6.283185307179586476925286766559 * u2;
+    const double z = rad * std::cos(ang);
+    const double sz = sigma * z;
+    const double e = mu + sz;
+    return std::exp(e);
+}
+
+inline int64_t below(uint64_t r, uint64_t m) { return (int64_t)(((unsigned __int128)r * m) >> 64); }
+
+}  // namespace
+
+RMQ_API int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int32_t nthreads) {
+    if (!Q || q < 0 || n <= 0 || lr == 0 || lr > n || lr < -6) return RMQ_ERR_INVALID_ARG;
+    const uint64_t k1 = rng_key(seed, 1), k2 = rng_key(seed, 2), k3 = rng_key(seed, 3), k4 = rng_key(seed, 4);
+    double mu = 0.0, sigma = 1.0;
+    if (lr == -2) mu = 14.0 * 0.69314718055994530942;
+    if (lr == -3) mu = 6.0 * 0.69314718055994530942;
+    if (lr == -4) {
+        const int64_t sc = n / 256;
+        mu = std::log((double)(sc > 1 ? sc : 1));
+        sigma = 0.3;
+    }
+    if (lr == -5) {
+        const int64_t sc = n / 32768;
+        mu = std::log((double)(sc > 1 ? sc : 1));
+        sigma = 0.3;
+    }
+    auto work = [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const uint64_t ui = (uint64_t)i;
+            int64_t l, len;
+            if (lr > 0) {
+                len = lr;
+                l = below(rng_at(k1, ui), (uint64_t)(n - lr + 1));
+            } else {
+                l = below(rng_at(k1, ui), (uint64_t)n);
+                const int64_t room = (int64_t)n - l;
+                if (lr == -1) {
+                    len = 1 + below(rng_at(k2, ui), (uint64_t)room);
+                } else if (lr == -6) {
+                    len = room;
+                } else {
+                    double x = lognormal_len(k3, k4, ui, mu, sigma);
+                    if (!(x < 4.0e18)) x = 4.0e18;
+                    len = std::llround(x);
+                    if (len < 1) len = 1;
+                    if (len > room) len = room;
+                }
+            }
+            Q[2 * i] = (int32_t)l;
+            Q[2 * i + 1] = (int32_t)(l + len - 1);
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    if (q < 65536) nt = 1;
+    if (nt == 1) {
+        work(0, q);
+        return RMQ_OK;
+    }
+    std::vector<std::thread> pool;
+    const int64_t chunk = (q + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int64_t lo = t * chunk, hi = std::min<int64_t>(q, lo + chunk);
+        if (lo >= hi) break;
+        pool.emplace_back(work, lo, hi);
+    }
+    for (auto &th : pool) th.join();
+    return RMQ_OK;
+}
+
+RMQ_API const char *rmq_strerror(int code) {
+    switch (code) {
+    case RMQ_OK: return "ok";
+    case RMQ_ERR_INVALID_ARG: return "invalid argument";
+    case RMQ_ERR_OOM: return "out of device memory";
+    case RMQ_ERR_HIP: return "HIP runtime error";
+    case RMQ_ERR_DEVICE: return "handle used on a device other than the one it was built on";
+    case RMQ_ERR_BAD_QUERY: return "query with l < 0, l > r or r >= n";
+    default: return "unknown error";
+    }
+}
+
+RMQ_API const char *rmq_version(void) { return "rtxrmq-amd 0.1 (gfx950)"; }

@@ -1,0 +1,241 @@
+"""Host-side mirror of the reference's RTX_blocks operator over the C ABI.
+
+The reference's interface for this path is
+    template <typename T> T* rtx_rmq(int alg, int n, int bs, int q,
+                                     float* darray, int2* dquery, CmdArgs args)
+(src/rtx.h:24-177): build the block structure, run the query launch `reps`
+times with a synchronisation after each (src/rtx.h:150-155), copy the result
+to the host and report RMQ/s = q / (time per rep).  `rtx_rmq` below keeps
+that contract (same alg ids, same argument meaning, host array out); the
+`RMQIndex` class exposes the split build/query form of include/rmq.h.
+
+torch is only the device-memory and stream plumbing; every computation runs
+in librtxrmq.so.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+# algorithm ids, src/main.cu:17-32
+ALG_CPU_BASE = 0
+ALG_CPU_HRMQ = 1
+ALG_GPU_BASE = 2
+ALG_GPU_RTX_BLOCKS = 5
+ALG_CPU_BASE_IDX = 100
+ALG_CPU_HRMQ_IDX = 101
+ALG_GPU_BASE_IDX = 102
+ALG_GPU_RTX_BLOCKS_IDX = 105
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream) -> int | None:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream or None
+
+
+def _dev_f32(t, name: str):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float32 or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous float32 CUDA tensor")
+    return t
+
+
+def _dev_queries(t):
+    torch = _torch()
+    if (not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.int32
+            or not t.is_contiguous() or t.numel() % 2):
+        raise TypeError("queries must be a contiguous int32 CUDA tensor of {l, r} pairs")
+    return t
+
+
+def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0) -> N.QueryOpts:
+    return N.QueryOpts(short_max, group_lanes, long_wgs, 0)
+
+
+class RMQIndex:
+    """A built block index over a device array (rmq_build / rmq_query / rmq_free).
+
+    The array tensor is kept referenced: the edge-block scans read it.
+    """
+
+    def __init__(self, A, bs: int, stream=None):
+        self.A = _dev_f32(A, "A")
+        self.n = int(A.numel())
+        self.bs = int(bs)
+        h = ctypes.c_void_p()
+        N.check(N.lib().rmq_build(A.data_ptr(), self.n, self.bs, 0, _stream_ptr(stream), ctypes.byref(h)),
+                "rmq_build")
+        self._h = h
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def info(self) -> dict:
+        n, nb, byt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        bs, lev = ctypes.c_int32(), ctypes.c_int32()
+        N.check(N.lib().rmq_handle_info(self._h, ctypes.byref(n), ctypes.byref(bs), ctypes.byref(nb),
+                                        ctypes.byref(lev), ctypes.byref(byt)), "rmq_handle_info")
+        return {"n": n.value, "bs": bs.value, "nb": nb.value, "levels": lev.value, "bytes": byt.value}
+
+    def default_opts(self) -> N.QueryOpts:
+        o = N.QueryOpts()
+        N.check(N.lib().rmq_query_opts_default(self._h, ctypes.byref(o)), "rmq_query_opts_default")
+        return o
+
+    def _outs(self, q: int, want_value: bool, want_index: bool, out_val, out_idx):
+        torch = _torch()
+        dev = self.A.device
+        if want_value and out_val is None:
+            out_val = torch.empty(q, dtype=torch.float32, device=dev)
+        if want_index and out_idx is None:
+            out_idx = torch.empty(q, dtype=torch.int32, device=dev)
+        return (out_val if want_value else None), (out_idx if want_index else None)
+
+    def query(self, Q, want_value: bool = True, want_index: bool = True, opts: N.QueryOpts | None = None,
+              out_val=None, out_idx=None, stream=None):
+        Q = _dev_queries(Q)
+        q = Q.numel() // 2
+        v, i = self._outs(q, want_value, want_index, out_val, out_idx)
+        rc = N.lib().rmq_query_ex(self._h, Q.data_ptr(), q, v.data_ptr() if v is not None else None,
+                                  i.data_ptr() if i is not None else None,
+                                  ctypes.byref(opts) if opts is not None else None, _stream_ptr(stream))
+        N.check(rc, "rmq_query")
+        return v, i
+
+    def query_timed(self, Q, want_value: bool = True, want_index: bool = True, opts: N.QueryOpts | None = None,
+                    out_val=None, out_idx=None, stream=None):
+        Q = _dev_queries(Q)
+        q = Q.numel() // 2
+        v, i = self._outs(q, want_value, want_index, out_val, out_idx)
+        t = N.QueryTimes()
+        rc = N.lib().rmq_query_timed(self._h, Q.data_ptr(), q, v.data_ptr() if v is not None else None,
+                                     i.data_ptr() if i is not None else None,
+                                     ctypes.byref(opts) if opts is not None else None, _stream_ptr(stream),
+                                     ctypes.byref(t))
+        N.check(rc, "rmq_query_timed")
+        return v, i, {"short_ms": t.short_ms, "long_ms": t.long_ms, "total_ms": t.total_ms, "n_long": t.n_long}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().rmq_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def gpu_base(A, Q, want_value: bool = True, want_index: bool = True, stream=None):
+    """Brute force on the device (algs 2/102; the --check oracle)."""
+    torch = _torch()
+    A = _dev_f32(A, "A")
+    Q = _dev_queries(Q)
+    q = Q.numel() // 2
+    v = torch.empty(q, dtype=torch.float32, device=A.device) if want_value else None
+    i = torch.empty(q, dtype=torch.int32, device=A.device) if want_index else None
+    N.check(N.lib().rmq_gpu_base(A.data_ptr(), A.numel(), Q.data_ptr(), q, v.data_ptr() if v is not None else None,
+                                 i.data_ptr() if i is not None else None, _stream_ptr(stream)), "rmq_gpu_base")
+    return v, i
+
+
+def check_queries(Q, n: int, stream=None) -> int:
+    """Number of queries with l < 0, l > r or r >= n (synchronising)."""
+    Q = _dev_queries(Q)
+    bad = ctypes.c_int64()
+    rc = N.lib().rmq_query_check(Q.data_ptr(), Q.numel() // 2, n, _stream_ptr(stream), ctypes.byref(bad))
+    if rc not in (N.RMQ_OK, N.RMQ_ERR_BAD_QUERY):
+        N.check(rc, "rmq_query_check")
+    return bad.value
+
+
+def gen_array(n: int, seed: int, kind: str = "uniform", device="cuda", stream=None):
+    """Device array from the counter-based generator (create_random_array_dev, src/rand.cuh:23-43)."""
+    torch = _torch()
+    A = torch.empty(n, dtype=torch.float32, device=device)
+    N.check(N.lib().rmq_gen_array(A.data_ptr(), n, seed, N.GEN_KINDS[kind], _stream_ptr(stream)), "rmq_gen_array")
+    return A
+
+
+def gen_array_host(n: int, seed: int, kind: str = "uniform") -> np.ndarray:
+    A = np.empty(n, np.float32)
+    N.check(N.lib().rmq_gen_array_host(A.ctypes.data, n, seed, N.GEN_KINDS[kind]), "rmq_gen_array_host")
+    return A
+
+
+def gen_queries(q: int, n: int, lr: int, seed: int, nthreads: int = 0) -> np.ndarray:
+    """Host queries (random_queries_par_cpu, src/rand.cuh:179-201); shape (q, 2) int32."""
+    Q = np.empty((q, 2), np.int32)
+    N.check(N.lib().rmq_gen_queries_host(Q.ctypes.data, q, n, lr, seed, nthreads), "rmq_gen_queries_host")
+    return Q
+
+
+@dataclass
+class RunResult:
+    """What the reference prints and writes per run (src/rtx.h:160-162, src/tools.h:299-336)."""
+    output: np.ndarray
+    time_ms: float                 # total over reps
+    reps: int
+    q: int
+    construction_ms: float
+    index_bytes: int
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def rmq_per_s(self) -> float:
+        return self.q / (self.time_ms / self.reps / 1000.0)
+
+    @property
+    def ns_per_rmq(self) -> float:
+        return self.time_ms / self.reps * 1e6 / self.q
+
+
+def rtx_rmq(alg: int, n: int, bs: int, q: int, darray, dquery, reps: int = 10, opts: N.QueryOpts | None = None,
+            stream=None) -> RunResult:
+    """Same contract as the reference's rtx_rmq<T> (src/rtx.h:24-177) for algs 5 and 105.
+
+    Builds the index, runs `reps` query passes each followed by a device
+    synchronisation (src/rtx.h:151-154), and returns the host output: float32
+    minima for alg 5, leftmost int32 indices for alg 105.
+    """
+    torch = _torch()
+    if alg not in (ALG_GPU_RTX_BLOCKS, ALG_GPU_RTX_BLOCKS_IDX):
+        raise ValueError(f"rtx_rmq handles algs 5 and 105, not {alg}")
+    if darray.numel() != n or dquery.numel() != 2 * q:
+        raise ValueError("n / q do not match the device buffers")
+    want_value = alg < 100
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = RMQIndex(darray, bs, stream=stream)
+    torch.cuda.synchronize()
+    construction_ms = (time.perf_counter() - t0) * 1e3
+    try:
+        out_v, out_i = index._outs(q, want_value, not want_value, None, None)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            index.query(dquery, want_value, not want_value, opts=opts, out_val=out_v, out_idx=out_i, stream=stream)
+            torch.cuda.synchronize()
+        time_ms = (time.perf_counter() - t0) * 1e3
+        out = (out_v if want_value else out_i).cpu().numpy()
+        return RunResult(out, time_ms, reps, q, construction_ms, index.info()["bytes"])
+    finally:
+        index.close()

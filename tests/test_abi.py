@@ -1,0 +1,81 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/rmq.h declares, validates arguments before touching the device, and
+its host-side generators agree with the oracle's."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import rtxrmq_amd as R
+from rtxrmq_amd import _native as N
+from oracle import oracle as O
+
+
+def test_library_loads_and_exports_header():
+    lib = R.lib()
+    names = R.header_functions()
+    assert {"rmq_build", "rmq_query", "rmq_free"} <= set(names)
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in include/rmq.h but not exported"
+    dyn = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in dyn.splitlines() if line.strip()}
+    assert set(names) <= exported
+
+
+def test_strerror_and_version():
+    assert R.strerror(R.RMQ_OK) == "ok"
+    for code in (R.RMQ_ERR_INVALID_ARG, R.RMQ_ERR_OOM, R.RMQ_ERR_HIP, R.RMQ_ERR_DEVICE, R.RMQ_ERR_BAD_QUERY):
+        assert R.strerror(code) not in ("ok", "unknown error")
+    assert b"gfx950" in R.lib().rmq_version()
+
+
+def test_argument_validation_before_device_use():
+    lib = R.lib()
+    h = ctypes.c_void_p()
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.rmq_build(None, 10, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(fake, 0, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(fake, 10, 0, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(fake, 1 << 31, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(fake, 10, 4, 7, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(fake, 10, 4, 0, None, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_query(None, fake, 4, fake, fake, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_gpu_base(None, 10, fake, 1, fake, None, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_gpu_base(fake, 10, fake, 1, None, None, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_gen_array(None, 10, 1, 0, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_gen_array(fake, 10, 1, 9, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_query_check(fake, 1, 10, None, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_info(None, None, None, None, None, None) == R.RMQ_ERR_INVALID_ARG
+    lib.rmq_free(None)  # no-op
+
+
+def test_query_generator_rejects_bad_specs():
+    Q = np.empty((4, 2), np.int32)
+    lib = R.lib()
+    for n, lr in [(0, -1), (10, 0), (10, 11), (10, -7)]:
+        assert lib.rmq_gen_queries_host(Q.ctypes.data, 4, n, lr, 1, 1) == R.RMQ_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("lr", [-1, -2, -3, -4, -5, -6, 1, 1000])
+def test_product_query_generator_matches_oracle(lr):
+    a = R.gen_queries(200000, 10 ** 8, lr, 1321, nthreads=4)
+    b = O.gen_queries(200000, 10 ** 8, lr, 1321, nt=3)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "ties16", "desc", "const"])
+def test_product_array_generator_matches_oracle(kind):
+    a = R.gen_array_host(100003, 42, kind)
+    b = O.gen_array(100003, 42, kind)
+    np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def test_product_path_has_no_cpu_fallback():
+    """The product package never imports the oracle."""
+    import pathlib
+    import re
+    pkg = pathlib.Path(N.PKG)
+    pat = re.compile(r"(^\s*(from|import)\s+oracle)|liboracle|libhrmq|_ref/", re.M)
+    for f in list(pkg.rglob("*.py")) + list(pkg.rglob("*.hip")) + list(pkg.rglob("*.cpp")):
+        assert not pat.search(f.read_text()), f"{f} reaches into oracle/"

@@ -1,0 +1,209 @@
+"""Parity of the HIP engine (algs 5/105 through the C ABI) with the CPU oracle.
+
+Bar: bit-exact.  Values are compared as int32 bit patterns (so -0.0 vs +0.0
+counts), indices exactly, leftmost tie rule (CPU BASE, src/cpu_methods.h:7-53).
+Small cases go against brute force and the golden fixtures; the BASELINE
+sizes (n=1e8, q=2^26; n=2^27 stress arrays) go against the O(1) fast oracle
+plus size-independent properties (l <= idx <= r, A[idx] == value).
+"""
+import numpy as np
+import pytest
+import torch
+
+import rtxrmq_amd as R
+from conftest import golden_cases, load_golden
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, gpu):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(gpu)
+
+
+def run_engine(A_np, Q_np, bs, gpu, opts=None, A_dev=None):
+    A = A_dev if A_dev is not None else _dev(A_np, gpu)
+    Q = _dev(Q_np.reshape(-1, 2), gpu)
+    with R.RMQIndex(A, bs) as ix:
+        v, i = ix.query(Q, opts=opts)
+        torch.cuda.synchronize()
+        return v.cpu().numpy(), i.cpu().numpy()
+
+
+def assert_exact(val, idx, ev, ei, what=""):
+    bad = np.nonzero((idx != ei) | (val.view(np.int32) != ev.view(np.int32)))[0]
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}: got {idx[bad[:5]]} want {ei[bad[:5]]}"
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", golden_cases())
+@pytest.mark.parametrize("bs", [1, 3, 4, 16, 1 << 15])
+def test_golden(name, bs, gpu):
+    g = load_golden(name)
+    v, i = run_engine(g["A"], g["Q"], bs, gpu)
+    assert_exact(v, i, g["val"], g["idx"], f"{name} bs={bs}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 130, 257])
+@pytest.mark.parametrize("bs", [1, 2, 3, 5, 8, 64, 1000])
+def test_exhaustive_small(n, bs, gpu):
+    rng = np.random.default_rng(n * 7919 + bs)
+    A = rng.integers(0, 4, n).astype(np.float32)
+    Q = np.array([(l, r) for l in range(n) for r in range(l, n)], np.int32)
+    ev, ei = O.rmq_brute(A, Q)
+    v, i = run_engine(A, Q, bs, gpu)
+    assert_exact(v, i, ev, ei, f"n={n} bs={bs}")
+
+
+@pytest.fixture(scope="module")
+def mixed_case():
+    A = O.gen_array(200003, 11, "ties16")
+    Q = np.concatenate([O.gen_queries(20000, A.size, lr, 3) for lr in (-1, -2, -3, 1, 5)])
+    ev, ei = O.rmq_brute(A, Q)
+    return A, Q, ev, ei
+
+
+@pytest.mark.parametrize("group_lanes", [4, 8, 16, 64])
+@pytest.mark.parametrize("short_max", [1, 7, 64, 1024, 1 << 30])
+def test_tuning_knobs_do_not_change_answers(group_lanes, short_max, mixed_case, gpu):
+    A, Q, ev, ei = mixed_case
+    opts = R.query_opts(short_max=short_max, group_lanes=group_lanes)
+    v, i = run_engine(A, Q, 1000, gpu, opts=opts)
+    assert_exact(v, i, ev, ei, f"G={group_lanes} short_max={short_max}")
+
+
+def test_config1_plain_small(gpu):
+    """n=2^20 U[0,1), q=2^20 uniform lengths, bs=2^15: engine vs fast oracle
+    (all queries), vs CPU BASE brute force and GPU BASE (subsample), vs the
+    reference HRMQ values when it is built."""
+    n, q, bs = 1 << 20, 1 << 20, 1 << 15
+    A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
+    A = A_dev.cpu().numpy()
+    np.testing.assert_array_equal(A.view(np.int32), O.gen_array(n, 1321).view(np.int32))
+    Q = R.gen_queries(q, n, -1, 1321)
+    v, i = run_engine(A, Q, bs, gpu, A_dev=A_dev)
+    f = O.FastOracle(A, nt=0)
+    fv, fi = f.query(Q)
+    f.close()
+    assert_exact(v, i, fv, fi, "config1 vs fast oracle")
+    sub = np.random.default_rng(0).choice(q, 3000, replace=False)
+    bv, bi = O.rmq_brute(A, Q[sub])
+    assert_exact(v[sub], i[sub], bv, bi, "config1 vs CPU BASE")
+    gv, gi = R.gpu_base(A_dev, _dev(Q[sub], gpu))
+    assert_exact(v[sub], i[sub], gv.cpu().numpy(), gi.cpu().numpy(), "config1 vs GPU BASE")
+    if O.HRMQ.available():
+        h = O.HRMQ(A)
+        hv, _ = h.query(Q)
+        h.close()
+        np.testing.assert_array_equal(v.view(np.int32), hv.view(np.int32))
+
+
+@pytest.mark.parametrize("lr", [-1, -2, -3])
+def test_headline_full_size(lr, gpu):
+    """n=1e8, q=2^26, bs=2^15, each length distribution: every answer checked."""
+    n, q, bs = 100_000_000, 1 << 26, 1 << 15
+    A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
+    Q = R.gen_queries(q, n, lr, 1321 + lr)
+    Q_dev = _dev(Q, gpu)
+    with R.RMQIndex(A_dev, bs) as ix:
+        v, i = ix.query(Q_dev)
+        torch.cuda.synchronize()
+        v, i = v.cpu().numpy(), i.cpu().numpy()
+    A = A_dev.cpu().numpy()
+    del Q_dev
+    assert np.all(i >= Q[:, 0]) and np.all(i <= Q[:, 1])
+    np.testing.assert_array_equal(A[i].view(np.int32), v.view(np.int32))
+    f = O.FastOracle(A)
+    fv, fi = f.query(Q)
+    f.close()
+    assert_exact(v, i, fv, fi, f"headline lr={lr}")
+
+
+@pytest.mark.parametrize("kind", ["ties16", "desc"])
+@pytest.mark.parametrize("bs", [1 << 10, 1 << 12, 1 << 15, 1 << 18])
+def test_stress_n2_27(kind, bs, gpu):
+    """n=2^27 heavy-tie and strictly descending arrays, fixed lengths
+    {1, 2^8, 2^16, n-l}, block sizes across the sweep (2^20 queries per length)."""
+    n = 1 << 27
+    A_dev = R.gen_array(n, 99, kind, device=gpu)
+    A = A_dev.cpu().numpy()
+    f = O.FastOracle(A)
+    with R.RMQIndex(A_dev, bs) as ix:
+        for lr in (1, 1 << 8, 1 << 16, -6):
+            Q = R.gen_queries(1 << 20, n, lr, 7)
+            v, i = ix.query(_dev(Q, gpu))
+            torch.cuda.synchronize()
+            fv, fi = f.query(Q)
+            assert_exact(v.cpu().numpy(), i.cpu().numpy(), fv, fi, f"{kind} bs={bs} lr={lr}")
+    f.close()
+
+
+def test_value_only_and_index_only(gpu):
+    A = O.gen_array(100000, 5, "ties16")
+    Q = O.gen_queries(50000, A.size, -2, 5)
+    ev, ei = O.rmq_brute(A, Q)
+    with R.RMQIndex(_dev(A, gpu), 4096) as ix:
+        v, none = ix.query(_dev(Q, gpu), want_value=True, want_index=False)
+        none2, i = ix.query(_dev(Q, gpu), want_value=False, want_index=True)
+        torch.cuda.synchronize()
+    assert none is None and none2 is None
+    np.testing.assert_array_equal(v.cpu().numpy().view(np.int32), ev.view(np.int32))
+    np.testing.assert_array_equal(i.cpu().numpy(), ei)
+
+
+def test_invalid_queries_are_flagged_not_faulted(gpu):
+    A = O.gen_array(1000, 5)
+    Q = np.array([[0, 999], [5, 4], [-1, 3], [10, 1000], [3, 3], [999, 999]], np.int32)
+    v, i = run_engine(A, Q, 64, gpu)
+    assert list(i[[1, 2, 3]]) == [-1, -1, -1] and np.all(np.isnan(v[[1, 2, 3]]))
+    ev, ei = O.rmq_brute(A, Q[[0, 4, 5]])
+    assert_exact(v[[0, 4, 5]], i[[0, 4, 5]], ev, ei)
+    assert R.check_queries(_dev(Q, gpu), 1000) == 3
+
+
+def test_empty_and_single(gpu):
+    A = _dev(np.array([2.5], np.float32), gpu)
+    with R.RMQIndex(A, 7) as ix:
+        v, i = ix.query(torch.empty((0, 2), dtype=torch.int32, device=gpu))
+        assert v.numel() == 0 and i.numel() == 0
+        v, i = ix.query(_dev(np.array([[0, 0]], np.int32), gpu))
+        assert i.item() == 0 and v.item() == 2.5
+        info = ix.info()
+        assert info["nb"] == 1 and info["levels"] == 1
+
+
+def test_device_generator_matches_host(gpu):
+    for kind in ("uniform", "ties16", "desc", "const"):
+        d = R.gen_array(1 << 20, 1234, kind, device=gpu).cpu().numpy()
+        np.testing.assert_array_equal(d.view(np.int32), R.gen_array_host(1 << 20, 1234, kind).view(np.int32))
+
+
+def test_concurrent_streams_same_handle(gpu):
+    A = O.gen_array(1 << 20, 8)
+    Qs = [O.gen_queries(1 << 16, A.size, lr, 8) for lr in (-1, -2, -3)]
+    A_dev = _dev(A, gpu)
+    streams = [torch.cuda.Stream() for _ in Qs]
+    with R.RMQIndex(A_dev, 1 << 12) as ix:
+        torch.cuda.synchronize()
+        outs = []
+        for s, Q in zip(streams, Qs):
+            with torch.cuda.stream(s):
+                Qd = _dev(Q, gpu)
+                outs.append((Qd,) + ix.query(Qd, stream=s))
+        torch.cuda.synchronize()
+        for (Qd, v, i), Q in zip(outs, Qs):
+            ev, ei = O.rmq_brute(A, Q)
+            assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei)
+
+
+def test_rtx_rmq_mirror(gpu):
+    n, q, bs = 1 << 16, 1 << 14, 1 << 10
+    A_dev = R.gen_array(n, 3, device=gpu)
+    Q = R.gen_queries(q, n, -2, 3)
+    A = A_dev.cpu().numpy()
+    ev, ei = O.rmq_brute(A, Q)
+    r5 = R.rtx_rmq(R.ALG_GPU_RTX_BLOCKS, n, bs, q, A_dev, _dev(Q, gpu), reps=3)
+    r105 = R.rtx_rmq(R.ALG_GPU_RTX_BLOCKS_IDX, n, bs, q, A_dev, _dev(Q, gpu), reps=3)
+    np.testing.assert_array_equal(r5.output.view(np.int32), ev.view(np.int32))
+    np.testing.assert_array_equal(r105.output, ei)
+    assert r5.rmq_per_s > 0 and r105.reps == 3
