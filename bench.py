@@ -21,6 +21,7 @@ over queries as src/cpu_methods.h:104-110, on a bounded sample of each batch.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -81,9 +82,11 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
     if not O.HRMQ.available():
         log("cpu_baseline: oracle/_ref/libhrmq.so missing; skipping")
         return None
+    log(f"cpu_baseline: building reference HRMQ over n={A_host.size} (single thread)")
     t0 = time.perf_counter()
     h = O.HRMQ(A_host)
     build_s = time.perf_counter() - t0
+    log(f"cpu_baseline: built in {build_s:.2f} s; timing queries on {nt} threads")
     total_q, total_t, per = 0, 0.0, {}
     for lr, Q in queries.items():
         pilot = Q[: 1 << 16]
@@ -96,6 +99,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
         h.query(sample, nt=nt, want_val=True, want_idx=True)
         dt = time.perf_counter() - t0
         per[DIST_NAMES[lr]] = {"rmq_per_s": m / dt, "queries": m}
+        log(f"cpu_baseline: {DIST_NAMES[lr]} {m / dt:.3e} RMQ/s over {m} queries")
         total_q += m
         total_t += dt
     size = h.size_bytes()
@@ -113,6 +117,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
 
 
 def main() -> int:
+    faulthandler.enable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -199,6 +204,7 @@ def main() -> int:
     workload = f"n={n} q={q} bs={bs} lr=-1,-2,-3 value+index"
     traffic = load_traffic("k_query_long", workload)
 
+    log(f"bench: long kernel {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
     cpu = None
     if not args.no_cpu_baseline:
         A_host = A.cpu().numpy()

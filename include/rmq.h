@@ -16,8 +16,8 @@
  * Queries are int32 pairs {l, r}, inclusive, 0 <= l <= r < n, the layout of
  * the reference's int2 queries (src/rand.cuh:149-150).  Inputs must not
  * contain NaN (the reference's scan order is undefined for NaN as well).
- * Invalid queries never fault: they produce value NaN and index -1 and are
- * counted (rmq_query_ex's n_invalid).
+ * Invalid queries never fault: they produce value NaN and index -1
+ * (rmq_query_check counts them).
  *
  * Errors: every call returns RMQ_OK (0) or a negative RMQ_ERR_* code; no
  * exception or abort crosses the ABI (the reference's CUDA_CHECK prints and
@@ -65,7 +65,6 @@ typedef struct rmq_query_times {
     float short_ms; /* binning pass fused with the short-range group scans */
     float long_ms;  /* long-range wave scans (edge blocks + table)          */
     float total_ms; /* from the first launch to the last                   */
-    int32_t n_long; /* queries that went to the long-range kernel          */
 } rmq_query_times_t;
 
 /* ---- index build: replaces gen_vertices_blocks_dev + gen_triangles_dev +
@@ -83,8 +82,9 @@ int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream
  *      __raygen__rmq_blocks / __raygen__rmq_blocks_idx (src/rtx.h:150-155,
  *      src/rtx_kernels.cu:85-180,350-449).  d_Q holds q pairs {l, r};
  *      d_val (alg 5) and/or d_idx (alg 105) receive q results; either may be
- *      NULL but not both.  Stream-ordered and asynchronous; the handle is
- *      read-only, so concurrent queries on different streams are safe. ---- */
+ *      NULL but not both.  Stream-ordered and asynchronous; allocates no
+ *      device memory; the handle is read-only, so concurrent queries on
+ *      different streams are safe. ---- */
 int rmq_query(const rmq_handle_t *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx, hipStream_t s);
 
 /* rmq_query with tuning options (NULL = defaults). */

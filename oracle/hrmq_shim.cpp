@@ -10,8 +10,30 @@
 // (index): an OpenMP parallel-for over queries calling queryRMQ(l, r), with
 // the array read as int32 bit patterns (src/main.cu:118,123).
 #include <cstdint>
+#include <cstdlib>
+#include <new>
 #include <omp.h>
 #include "RMQRMM64.h"
+
+// The reference allocates its bit vector without clearing it
+// (hrmq/RMQRMM64.cpp:53, `P = new ulong[lenP]`) and sets only the first
+// 2(n+1) bits, so the padding bits of the last word -- and any table word it
+// never writes -- hold whatever the heap held before.  In a long-running
+// process with recycled heap memory (or under MALLOC_PERTURB_) queries then
+// read garbage and can fault.  These replacements, bound inside this library
+// only (-Bsymbolic in oracle/Makefile), zero every allocation the library
+// makes, so HRMQ's answers no longer depend on heap contents.
+#define HRMQ_LOCAL __attribute__((visibility("hidden")))
+HRMQ_LOCAL void *operator new(std::size_t sz) {
+    void *p = std::calloc(1, sz ? sz : 1);
+    if (!p) throw std::bad_alloc();
+    return p;
+}
+HRMQ_LOCAL void *operator new[](std::size_t sz) { return operator new(sz); }
+HRMQ_LOCAL void operator delete(void *p) noexcept { std::free(p); }
+HRMQ_LOCAL void operator delete[](void *p) noexcept { std::free(p); }
+HRMQ_LOCAL void operator delete(void *p, std::size_t) noexcept { std::free(p); }
+HRMQ_LOCAL void operator delete[](void *p, std::size_t) noexcept { std::free(p); }
 
 using namespace rmqrmm;
 

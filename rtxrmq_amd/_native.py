@@ -45,8 +45,7 @@ class QueryOpts(ctypes.Structure):
 
 class QueryTimes(ctypes.Structure):
     """rmq_query_times_t"""
-    _fields_ = [("short_ms", ctypes.c_float), ("long_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
-                ("n_long", _i32)]
+    _fields_ = [("short_ms", ctypes.c_float), ("long_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
 
 
 _LIB = None

@@ -1,0 +1,111 @@
+// power.h -- GPU power sampler for --save-power.
+//
+// The reference polls NVML every 100 ms on a pthread and writes one row per
+// sample: timestep, power (W), accumulated energy (J), average power, dt,
+// accumulated time (common/nvmlPower.cpp:37-84).  Here the amdgpu driver's
+// hwmon sensor (power1_average or power1_input, microwatts) of the card whose
+// PCI bus id matches the HIP device is read instead; rows keep the same
+// columns so the reference's RMQ/J scripts read them unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cctype>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <dirent.h>
+#include <string>
+#include <thread>
+
+class PowerSampler {
+   public:
+    ~PowerSampler() { stop(); }
+
+    // Starts sampling every period_ms into `file` (default ../results/power-<alg>.dat).
+    bool start(int dev, const std::string &file, const char *alg, int period_ms = 100) {
+        stop();
+        sensor_ = find_sensor(dev);
+        if (sensor_.empty()) {
+            fprintf(stderr, "--save-power: no readable amdgpu power sensor for device %d; skipping\n", dev);
+            return false;
+        }
+        std::string path = file;
+        if (path.empty()) {
+            path = "../results/power-";
+            for (const char *c = alg; *c; ++c) path += std::isalnum((unsigned char)*c) ? *c : '_';
+            path += ".dat";
+        }
+        out_ = fopen(path.c_str(), "w+");
+        if (!out_) return false;
+        fprintf(out_, "%-15s %-15s %-15s %-15s %-15s %-15s\n", "#timestep", "power", "acc-energy", "avg-power", "dt",
+                "acc-time");
+        running_ = true;
+        thread_ = std::thread([this, period_ms] { loop(period_ms); });
+        return true;
+    }
+
+    void stop() {
+        if (!running_) return;
+        running_ = false;
+        if (thread_.joinable()) thread_.join();
+        if (out_) fclose(out_);
+        out_ = nullptr;
+        printf("power: %.3f J over %.3f s, average %.2f W\n", energy_j_, time_s_, time_s_ > 0 ? energy_j_ / time_s_ : 0.0);
+    }
+
+   private:
+    static double read_watts(const std::string &sensor) {
+        FILE *f = fopen(sensor.c_str(), "r");
+        if (!f) return -1;
+        long long uw = -1;
+        if (fscanf(f, "%lld", &uw) != 1) uw = -1;
+        fclose(f);
+        return uw < 0 ? -1 : uw / 1e6;
+    }
+
+    static std::string find_sensor(int dev) {
+        char bus[64] = {0};
+        if (hipDeviceGetPCIBusId(bus, sizeof(bus), dev) != hipSuccess) return "";
+        for (char *c = bus; *c; ++c) *c = (char)std::tolower((unsigned char)*c);
+        const std::string base = std::string("/sys/bus/pci/devices/") + bus + "/hwmon";
+        DIR *d = opendir(base.c_str());
+        if (!d) return "";
+        std::string found;
+        while (dirent *e = readdir(d)) {
+            if (std::string(e->d_name).rfind("hwmon", 0) != 0) continue;
+            for (const char *name : {"power1_average", "power1_input"}) {
+                const std::string p = base + "/" + e->d_name + "/" + name;
+                if (read_watts(p) >= 0) { found = p; break; }
+            }
+            if (!found.empty()) break;
+        }
+        closedir(d);
+        return found;
+    }
+
+    void loop(int period_ms) {
+        using clk = std::chrono::steady_clock;
+        auto t1 = clk::now();
+        int step = 0;
+        while (running_) {
+            std::this_thread::sleep_for(std::chrono::milliseconds(period_ms));
+            const auto t2 = clk::now();
+            const double dt = std::chrono::duration<double>(t2 - t1).count();
+            t1 = t2;
+            const double w = read_watts(sensor_);
+            if (w < 0) continue;
+            ++step;
+            time_s_ += dt;
+            energy_j_ += w * dt;
+            fprintf(out_, "%-15i %-15f %-15f %-15f %-15f %-15f\n", step, w, energy_j_, energy_j_ / time_s_, dt, time_s_);
+        }
+    }
+
+    std::string sensor_;
+    FILE *out_ = nullptr;
+    std::atomic<bool> running_{false};
+    std::thread thread_;
+    double energy_j_ = 0, time_s_ = 0;
+};

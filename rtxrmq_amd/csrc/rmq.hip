@@ -21,7 +21,6 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -190,35 +189,49 @@ __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__
 
 // ---------------------------------------------------------------------------
 // query kernels
+//
+// Binning is done per tile of 64 consecutive queries, in registers: lane j of
+// a wave loads query t*64+j, and two ballots split the tile into short ranges
+// (at most short_max elements) and long ones.  The short kernel answers the
+// short ranges of every tile, several per wave; the long kernel answers the
+// long ranges of every tile, one whole wave each.  Both read the query array
+// once, so binning costs 8 bytes per query per kernel and no atomics, no
+// queue and no workspace: each query writes only its own output slot.
 // ---------------------------------------------------------------------------
-
-struct QueryCounters {
-    uint32_t n_long;
-    uint32_t n_invalid;
-};
-
-__global__ void k_zero_counters(QueryCounters *c) {
-    if (threadIdx.x == 0) {
-        c->n_long = 0;
-        c->n_invalid = 0;
-    }
-}
 
 __device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
     if (out_v) out_v[qi] = v;
     if (out_i) out_i[qi] = i;
 }
 
-// Binning pass fused with the short-range scans.  Each wave takes tiles of 64
-// consecutive queries.  Invalid queries get (NaN, -1).  Ranges of more than
-// short_max elements are appended to long_ids (one atomic per wave); the
-// short ones are staged in LDS and answered G lanes per query, 64/G queries
-// per round, every lane taking part in the DPP reduction of each round.
+struct TileMasks {
+    int l, r;          // this lane's query
+    uint64_t shorts;   // lanes holding a valid range of at most short_max elements
+    uint64_t longs;    // lanes holding a valid longer range
+};
+
+__device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64_t q, int n, int short_max, int64_t t,
+                                               int lane, bool flag_invalid, float *out_v, int32_t *out_i) {
+    const int64_t qi = t * kWave + lane;
+    const bool in = qi < q;
+    int l = 0, r = -1;
+    if (in) {
+        const int2 p = Q[qi];
+        l = p.x;
+        r = p.y;
+    }
+    const bool ok = in && l >= 0 && l <= r && r < n;
+    if (flag_invalid && in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    const bool is_short = ok && (r - l) < short_max;
+    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+}
+
+// Short ranges, G lanes per query, 64/G queries per round; every lane takes
+// part in the DPP reduction of every round.  Invalid queries get (NaN, -1).
 template <int G>
 __global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A, int n,
                                                      const int2 *__restrict__ Q, int64_t q, int short_max,
-                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i,
-                                                     int32_t *__restrict__ long_ids, QueryCounters *cnt) {
+                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i) {
     constexpr int QPR = kWave / G;  // queries per round
     __shared__ int s_l[kWavesPerWG][kWave];
     __shared__ int s_r[kWavesPerWG][kWave];
@@ -227,40 +240,19 @@ __global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
+    const int g = lane / G;
+    const int gl = lane & (G - 1);
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const int64_t qi = t * kWave + lane;
-        const bool in = qi < q;
-        int l = 0, r = -1;
-        if (in) {
-            const int2 p = Q[qi];
-            l = p.x;
-            r = p.y;
-        }
-        const bool ok = in && l >= 0 && l <= r && r < n;
-        if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
-        const uint64_t bad = __ballot(in && !ok);
-        if (bad && lane == 0) atomicAdd(&cnt->n_invalid, (uint32_t)__popcll(bad));
-        const bool is_short = ok && (r - l) < short_max;
-        const bool is_long = ok && !is_short;
-        const uint64_t lm = __ballot(is_long);
-        if (lm) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&cnt->n_long, (uint32_t)__popcll(lm));
-            base = __builtin_amdgcn_readfirstlane(base);
-            if (is_long) long_ids[base + mask_rank(lm)] = (int32_t)qi;
-        }
-        const uint64_t sm = __ballot(is_short);
-        if (is_short) {
-            const uint32_t k = mask_rank(sm);
-            s_l[wv][k] = l;
-            s_r[wv][k] = r;
-            s_q[wv][k] = (int)(qi - t * kWave);
+        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, true, out_v, out_i);
+        if ((m.shorts >> lane) & 1) {
+            const uint32_t k = mask_rank(m.shorts);
+            s_l[wv][k] = m.l;
+            s_r[wv][k] = m.r;
+            s_q[wv][k] = lane;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        const int n_short = __popcll(sm);
-        const int g = lane / G;
-        const int gl = lane & (G - 1);
+        const int n_short = __popcll(m.shorts);
         for (int base = 0; base < n_short; base += QPR) {
             const int s = base + g;
             const bool has = s < n_short;
@@ -269,8 +261,7 @@ __global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A
             int ql = 0;
             if (has) {
                 ql = s_l[wv][s];
-                const int qr = s_r[wv][s];
-                scan_range<G, 1>(A, ql, qr, gl, bv, bi);
+                scan_range<G, 1>(A, ql, s_r[wv][s], gl, bv, bi);
             }
             group_reduce<G>(bv, bi);
             if (has && gl == 0) {
@@ -282,43 +273,20 @@ __global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A
     }
 }
 
-// Whole-wave short path (group of 64 lanes): one query per round.
+// Short ranges with the whole wave per query (group of 64 lanes).
 __global__ __launch_bounds__(kWG) void k_query_short64(const float *__restrict__ A, int n,
                                                        const int2 *__restrict__ Q, int64_t q, int short_max,
-                                                       float *__restrict__ out_v, int32_t *__restrict__ out_i,
-                                                       int32_t *__restrict__ long_ids, QueryCounters *cnt) {
+                                                       float *__restrict__ out_v, int32_t *__restrict__ out_i) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const int64_t qi = t * kWave + lane;
-        const bool in = qi < q;
-        int l = 0, r = -1;
-        if (in) {
-            const int2 p = Q[qi];
-            l = p.x;
-            r = p.y;
-        }
-        const bool ok = in && l >= 0 && l <= r && r < n;
-        if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
-        const uint64_t bad = __ballot(in && !ok);
-        if (bad && lane == 0) atomicAdd(&cnt->n_invalid, (uint32_t)__popcll(bad));
-        const bool is_short = ok && (r - l) < short_max;
-        const bool is_long = ok && !is_short;
-        const uint64_t lm = __ballot(is_long);
-        if (lm) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&cnt->n_long, (uint32_t)__popcll(lm));
-            base = __builtin_amdgcn_readfirstlane(base);
-            if (is_long) long_ids[base + mask_rank(lm)] = (int32_t)qi;
-        }
-        uint64_t sm = __ballot(is_short);
-        while (sm) {
+        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, true, out_v, out_i);
+        for (uint64_t sm = m.shorts; sm; sm &= sm - 1) {
             const int src = __ffsll((unsigned long long)sm) - 1;
-            sm &= sm - 1;
-            const int ql = __builtin_amdgcn_readlane(l, src);
-            const int qr = __builtin_amdgcn_readlane(r, src);
+            const int ql = __builtin_amdgcn_readlane(m.l, src);
+            const int qr = __builtin_amdgcn_readlane(m.r, src);
             float bv = INFINITY;
             int bi = INT_MAX;
             scan_range<kWave, 2>(A, ql, qr, lane, bv, bi);
@@ -331,49 +299,52 @@ __global__ __launch_bounds__(kWG) void k_query_short64(const float *__restrict__
     }
 }
 
-// Long ranges: one wave per query, persistent grid striding over the queue.
-// lB == rB: one in-block scan (src/rtx_kernels.cu:116-135).  Otherwise the
-// left edge block from l, the right edge block up to r (:151-177) and, when
-// whole blocks lie between, the sparse table over block minima (:138-149).
-__global__ __launch_bounds__(kWG) void k_query_long(const float *__restrict__ A, int bs, int nb,
+// Long ranges, one whole wave per query.  lB == rB: one in-block scan
+// (src/rtx_kernels.cu:116-135).  Otherwise the left edge block from l, the
+// right edge block up to r (:151-177) and, when whole blocks lie between,
+// the sparse table over block minima (:138-149), combined with the index as
+// tie-break so the leftmost minimum wins.
+__global__ __launch_bounds__(kWG) void k_query_long(const float *__restrict__ A, int n, int bs, int nb,
                                                     const Pair *__restrict__ st, const int2 *__restrict__ Q,
-                                                    const int32_t *__restrict__ long_ids,
-                                                    const QueryCounters *__restrict__ cnt,
-                                                    float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+                                                    int64_t q, int short_max, float *__restrict__ out_v,
+                                                    int32_t *__restrict__ out_i) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave_global = blockIdx.x * kWavesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int wave_stride = gridDim.x * kWavesPerWG;
-    const int n_long = (int)cnt->n_long;
-    for (int k = wave_global; k < n_long; k += wave_stride) {
-        const int qi = long_ids[k];
-        const int2 p = Q[qi];
-        const int l = p.x, r = p.y;
-        const int lB = l / bs, rB = r / bs;
-        float bv = INFINITY;
-        int bi = INT_MAX;
-        float mv = INFINITY;
-        int mi = INT_MAX;
-        if (lB == rB) {
-            scan_range<kWave, kLongUnroll>(A, l, r, lane, bv, bi);
-        } else {
-            if (rB - lB > 1) {
-                const int a = lB + 1, c = rB - lB - 1;
-                const int lev = 31 - __clz(c);
-                const Pair *row = st + (int64_t)lev * nb;
-                const Pair x = row[a];
-                const Pair y = row[a + c - (1 << lev)];
-                const Pair m = (y.v < x.v) ? y : x;
-                mv = m.v;
-                mi = m.i;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
+    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, false, out_v, out_i);
+        for (uint64_t lm = m.longs; lm; lm &= lm - 1) {
+            const int src = __ffsll((unsigned long long)lm) - 1;
+            const int l = __builtin_amdgcn_readlane(m.l, src);
+            const int r = __builtin_amdgcn_readlane(m.r, src);
+            const int lB = l / bs, rB = r / bs;
+            float bv = INFINITY;
+            int bi = INT_MAX;
+            float mv = INFINITY;
+            int mi = INT_MAX;
+            if (lB == rB) {
+                scan_range<kWave, kLongUnroll>(A, l, r, lane, bv, bi);
+            } else {
+                if (rB - lB > 1) {
+                    const int a = lB + 1, c = rB - lB - 1;
+                    const int lev = 31 - __clz(c);
+                    const Pair *row = st + (int64_t)lev * nb;
+                    const Pair x = row[a];
+                    const Pair y = row[a + c - (1 << lev)];
+                    const Pair mm = (y.v < x.v) ? y : x;
+                    mv = mm.v;
+                    mi = mm.i;
+                }
+                scan_range<kWave, kLongUnroll>(A, l, lB * bs + bs - 1, lane, bv, bi);
+                scan_range<kWave, kLongUnroll>(A, rB * bs, r, lane, bv, bi);
             }
-            scan_range<kWave, kLongUnroll>(A, l, lB * bs + bs - 1, lane, bv, bi);
-            scan_range<kWave, kLongUnroll>(A, rB * bs, r, lane, bv, bi);
-        }
-        wave_reduce(bv, bi);
-        take_if_better(mv, mi, bv, bi);
-        if (lane == 0) {
-            if (bv == INFINITY) bi = l;
-            write_result(out_v, out_i, qi, bv, bi);
+            wave_reduce(bv, bi);
+            take_if_better(mv, mi, bv, bi);
+            if (lane == 0) {
+                if (bv == INFINITY) bi = l;
+                write_result(out_v, out_i, t * kWave + src, bv, bi);
+            }
         }
     }
 }
@@ -453,7 +424,6 @@ __global__ void k_gen_array(float *__restrict__ A, int64_t n, uint64_t key, int 
 // ---------------------------------------------------------------------------
 
 int g_num_cus = 0;
-std::once_flag g_pool_once;
 
 int device_cus() {
     if (g_num_cus > 0) return g_num_cus;
@@ -462,18 +432,6 @@ int device_cus() {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
     g_num_cus = cus;
     return cus;
-}
-
-// Keep stream-ordered allocations cached in the default pool between calls.
-void keep_pool_warm() {
-    std::call_once(g_pool_once, [] {
-        int dev = 0;
-        hipMemPool_t pool;
-        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-            uint64_t thr = UINT64_MAX;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-        }
-    });
 }
 
 int hip_rc(hipError_t e) {
@@ -533,80 +491,72 @@ int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts
     return RMQ_OK;
 }
 
-int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
-                 int32_t *long_ids, QueryCounters *cnt, hipStream_t s) {
+int short_grid(int64_t q) {
     const int64_t tiles = (q + kWave - 1) / kWave;
     const int64_t want = (tiles + kWavesPerWG - 1) / kWavesPerWG;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)device_cus() * 16));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)device_cus() * 16));
+}
+
+int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
+                 hipStream_t s) {
+    const int grid = short_grid(q);
     const int n = (int)h->n;
     switch (o.group_lanes) {
-    case 4: k_query_short<4><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
-    case 8: k_query_short<8><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
-    case 16: k_query_short<16><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
-    default: k_query_short64<<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx, long_ids, cnt); break;
+    case 4: k_query_short<4><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
+    case 8: k_query_short<8><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
+    case 16: k_query_short<16><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
+    default: k_query_short64<<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
     }
     return hip_rc(hipGetLastError());
 }
 
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
-                const int32_t *long_ids, const QueryCounters *cnt, hipStream_t s) {
-    const int64_t want = (q + kWavesPerWG - 1) / kWavesPerWG;
+                hipStream_t s) {
+    const int64_t tiles = (q + kWave - 1) / kWave;
+    const int64_t want = (tiles + kWavesPerWG - 1) / kWavesPerWG;
     int grid = o.long_wgs > 0 ? o.long_wgs : device_cus() * 8;
     grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, want));
-    k_query_long<<<grid, kWG, 0, s>>>(h->A, h->bs, (int)h->nb, h->st, Q, long_ids, cnt, val, idx);
+    k_query_long<<<grid, kWG, 0, s>>>(h->A, (int)h->n, h->bs, (int)h->nb, h->st, Q, q, o.short_max, val, idx);
     return hip_rc(hipGetLastError());
 }
 
 int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
                const rmq_query_opts_t *opts_in, hipStream_t s, rmq_query_times_t *times) {
-    if (!h || (!d_val && !d_idx) || q < 0 || (q > 0 && !d_Q)) return RMQ_ERR_INVALID_ARG;
-    if (q > INT32_MAX) return RMQ_ERR_INVALID_ARG;
-    int rc = check_device(h);
-    if (rc != RMQ_OK) return rc;
+    if (!h || q < 0 || q > INT32_MAX) return RMQ_ERR_INVALID_ARG;
     rmq_query_opts_t o;
-    rc = resolve_opts(h, opts_in, &o);
+    int rc = resolve_opts(h, opts_in, &o);
     if (rc != RMQ_OK) return rc;
-    if (q == 0) {
-        if (times) *times = rmq_query_times_t{0.f, 0.f, 0.f, 0};
+    if (q == 0) {  // nothing to read or write: empty buffers may be NULL
+        if (times) *times = rmq_query_times_t{0.f, 0.f, 0.f};
         return RMQ_OK;
     }
-    keep_pool_warm();
-    const size_t ws_bytes = 256 + sizeof(int32_t) * (size_t)q;
-    void *ws = nullptr;
-    RMQ_HIP(hipMallocAsync(&ws, ws_bytes, s));
-    QueryCounters *cnt = reinterpret_cast<QueryCounters *>(ws);
-    int32_t *long_ids = reinterpret_cast<int32_t *>(static_cast<char *>(ws) + 256);
+    if (!d_Q || (!d_val && !d_idx)) return RMQ_ERR_INVALID_ARG;
+    rc = check_device(h);
+    if (rc != RMQ_OK) return rc;
     const int2 *Q = reinterpret_cast<const int2 *>(d_Q);
-
+    if (!times) {
+        rc = launch_short(h, Q, q, o, d_val, d_idx, s);
+        if (rc == RMQ_OK) rc = launch_long(h, Q, q, o, d_val, d_idx, s);
+        return rc;
+    }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    if (times) {
-        for (auto &e : ev)
-            if ((rc = hip_rc(hipEventCreate(&e))) != RMQ_OK) break;
-    }
+    for (auto &e : ev)
+        if ((rc = hip_rc(hipEventCreate(&e))) != RMQ_OK) break;
+    if (rc == RMQ_OK) rc = hip_rc(hipEventRecord(ev[0], s));
+    if (rc == RMQ_OK) rc = launch_short(h, Q, q, o, d_val, d_idx, s);
+    if (rc == RMQ_OK) rc = hip_rc(hipEventRecord(ev[1], s));
+    if (rc == RMQ_OK) rc = launch_long(h, Q, q, o, d_val, d_idx, s);
+    if (rc == RMQ_OK) rc = hip_rc(hipEventRecord(ev[2], s));
+    if (rc == RMQ_OK) rc = hip_rc(hipEventSynchronize(ev[2]));
     if (rc == RMQ_OK) {
-        k_zero_counters<<<1, 64, 0, s>>>(cnt);
-        rc = hip_rc(hipGetLastError());
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        *times = rmq_query_times_t{a, b, a + b};
     }
-    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[0], s));
-    if (rc == RMQ_OK) rc = launch_short(h, Q, q, o, d_val, d_idx, long_ids, cnt, s);
-    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[1], s));
-    if (rc == RMQ_OK) rc = launch_long(h, Q, q, o, d_val, d_idx, long_ids, cnt, s);
-    if (rc == RMQ_OK && times) rc = hip_rc(hipEventRecord(ev[2], s));
-    if (rc == RMQ_OK && times) {
-        QueryCounters hc{0, 0};
-        rc = hip_rc(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
-        if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
-        if (rc == RMQ_OK) {
-            float a = 0.f, b = 0.f;
-            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
-            *times = rmq_query_times_t{a, b, a + b, (int32_t)hc.n_long};
-        }
-    }
-    const int rc_free = hip_rc(hipFreeAsync(ws, s));
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
-    return rc != RMQ_OK ? rc : rc_free;
+    return rc;
 }
 
 }  // namespace
@@ -715,8 +665,9 @@ RMQ_API int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts
 
 RMQ_API int rmq_gpu_base(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
                          hipStream_t s) {
-    if (!d_A || n <= 0 || n > INT32_MAX || q < 0 || (q > 0 && !d_Q) || (!d_val && !d_idx)) return RMQ_ERR_INVALID_ARG;
+    if (!d_A || n <= 0 || n > INT32_MAX || q < 0) return RMQ_ERR_INVALID_ARG;
     if (q == 0) return RMQ_OK;
+    if (!d_Q || (!d_val && !d_idx)) return RMQ_ERR_INVALID_ARG;
     const int bsz = 256;  // the reference uses BSIZE=1024 (src/main.cu:14); wave64 prefers 256
     k_gpu_base<<<(unsigned)((q + bsz - 1) / bsz), bsz, 0, s>>>(d_A, reinterpret_cast<const int2 *>(d_Q), q, (int)n,
                                                                d_val, d_idx);

@@ -125,7 +125,7 @@ class RMQIndex:
                                      ctypes.byref(opts) if opts is not None else None, _stream_ptr(stream),
                                      ctypes.byref(t))
         N.check(rc, "rmq_query_timed")
-        return v, i, {"short_ms": t.short_ms, "long_ms": t.long_ms, "total_ms": t.total_ms, "n_long": t.n_long}
+        return v, i, {"short_ms": t.short_ms, "long_ms": t.long_ms, "total_ms": t.total_ms}
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
