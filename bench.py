@@ -4,15 +4,21 @@
 
 A step is one query pass (alg 5 values + alg 105 indices in one launch set)
 over each of the three query batches: uniform lengths (lr=-1), lognormal
-medium (lr=-2) and lognormal small (lr=-3), 2^26 queries each, against one
-index built over n=1e8 float32 U[0,1) values with bs=2^15.  Inputs are
-resident in HBM before timing starts.  value = 3*q*K / (timed seconds).
+medium (lr=-2) and lognormal small (lr=-3), 2^26 queries each, over n=1e8
+float32 U[0,1) values.  Inputs are resident in HBM before timing starts.
+Two runs: the literal block size bs=2^15 with default thresholds
+(`fixed_bs_32768`), and the configuration the engine's autotuner picks per
+batch (block size, short-range threshold, lanes per query) from a timed
+2^20-query sample -- the way the reference's published runs tune --nb per
+distribution (scripts/perf-fulltest.sh:45-58).  value = 3*q*K / (timed
+seconds) of the tuned run; answers are bit-identical either way.
 
-roofline: the dominant kernel (k_query_long, the edge-block wave scans) --
-algorithmic bytes per query B = 8 (query) + 4*S (elements scanned) + T (two
-8-byte table entries when whole blocks lie between) + 8 (value + index out),
-summed over the queries that kernel processed, divided by its time measured
-with HIP events on the stream it runs on.
+roofline: the dominant kernel of the run -- algorithmic bytes per query
+B = 8 (query) + 4*S (elements scanned: the range for a short or in-block
+query, else both edge-block parts) + T (two 8-byte table entries when whole
+blocks lie between) + 8 (value + index out), summed over the queries that
+kernel processed, divided by its time measured with HIP events on the stream
+it runs on.
 
 cpu_baseline: the reference's HRMQ (RMQRMM64, built from the reference's own
 sources into oracle/_ref by oracle/Makefile) on this host's cores, OpenMP
@@ -116,6 +122,61 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
     }
 
 
+def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, warmup: int, label: str):
+    """Warmup, then `steps` timed steps; a step is one query pass per batch."""
+    def one_step():
+        res = {}
+        for lr in DISTS:
+            _, _, t = index_for[lr].query_timed(Q_dev[lr], out_val=out_v, out_idx=out_i, opts=opts_for[lr])
+            res[lr] = t
+        return res
+
+    for w in range(warmup):
+        one_step()
+        log(f"bench[{label}]: warmup {w + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    steps_out = []
+    for k in range(steps):
+        steps_out.append(one_step())
+        log(f"bench[{label}]: step {k + 1}/{steps}: " + ", ".join(
+            f"{DIST_NAMES[lr]} {s['total_ms']:.2f} ms" for lr, s in steps_out[-1].items()))
+    torch.cuda.synchronize()
+    return steps_out, time.perf_counter() - t_start
+
+
+def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for) -> dict:
+    """RMQ/s, per-distribution figures and the roofline of the dominant kernel."""
+    per_dist = {}
+    kernels = {"k_query_long": [0.0, 0.0], "k_query_short": [0.0, 0.0]}  # bytes, ms
+    for lr in DISTS:
+        lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs_for[lr], opts_for[lr].short_max)
+        ms = [s[lr]["total_ms"] for s in steps_out]
+        lms = [s[lr]["long_ms"] for s in steps_out]
+        sms = [s[lr]["short_ms"] for s in steps_out]
+        per_dist[DIST_NAMES[lr]] = {
+            "rmq_per_s": q / (np.mean(ms) / 1e3),
+            "ms_per_pass": float(np.mean(ms)),
+            "bs": bs_for[lr],
+            "opts": {"short_max": opts_for[lr].short_max, "group_lanes": opts_for[lr].group_lanes,
+                     "long_lanes": opts_for[lr].long_lanes},
+            "long_kernel_ms": float(np.mean(lms)),
+            "short_kernel_ms": float(np.mean(sms)),
+            "n_long": nl,
+            "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
+            "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
+        }
+        kernels["k_query_long"][0] += lb * len(steps_out)
+        kernels["k_query_long"][1] += float(np.sum(lms))
+        kernels["k_query_short"][0] += sb * len(steps_out)
+        kernels["k_query_short"][1] += float(np.sum(sms))
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    achieved = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
+    per_kernel = {k: {"ms": v[1], "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
+    return {"value": 3 * q * len(steps_out) / elapsed, "ms_per_step": elapsed / len(steps_out) * 1e3,
+            "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -124,7 +185,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100_000_000)
     ap.add_argument("--q", type=int, default=1 << 26)
-    ap.add_argument("--bs", type=int, default=1 << 15)
+    ap.add_argument("--bs", type=int, default=1 << 15, help="block size of the fixed-bs run")
+    ap.add_argument("--fixed-steps", type=int, default=2, help="timed steps of the fixed-bs run")
+    ap.add_argument("--tune-sample", type=int, default=1 << 20, help="queries per batch the autotuner times")
     ap.add_argument("--seed", type=int, default=1321)
     ap.add_argument("--cpu-seconds", type=float, default=1.5, help="wall seconds of HRMQ per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -140,8 +203,8 @@ def main() -> int:
     import rtxrmq_amd as R
 
     dev = torch.device("cuda:0")
-    n, q, bs = args.n, args.q, args.bs
-    log(f"bench: n={n} q={q} bs={bs} dists={DISTS} steps={args.steps} warmup={args.warmup}")
+    n, q = args.n, args.q
+    log(f"bench: n={n} q={q} dists={DISTS} steps={args.steps} warmup={args.warmup}")
 
     A = R.gen_array(n, args.seed, "uniform", device=dev)
     Q_host = {lr: R.gen_queries(q, n, lr, args.seed + 17 * (-lr)) for lr in DISTS}
@@ -150,61 +213,49 @@ def main() -> int:
     out_i = torch.empty(q, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
+    # --- the literal configuration: bs = 2^15, default thresholds -----------------
     t0 = time.perf_counter()
-    index = R.RMQIndex(A, bs)
+    fixed_index = R.RMQIndex(A, args.bs)
     torch.cuda.synchronize()
-    construction_ms = (time.perf_counter() - t0) * 1e3
-    opts = index.default_opts()
-    log(f"bench: index built in {construction_ms:.1f} ms ({index.info()})")
+    fixed_build_ms = (time.perf_counter() - t0) * 1e3
+    fixed_opts = fixed_index.default_opts()
+    fixed_steps, fixed_elapsed = timed_run(R, torch, {lr: fixed_index for lr in DISTS},
+                                           {lr: fixed_opts for lr in DISTS}, Q_dev, out_v, out_i,
+                                           args.fixed_steps, 1, f"bs={args.bs}")
+    fixed = summarise(Q_host, q, fixed_steps, fixed_elapsed, {lr: args.bs for lr in DISTS},
+                      {lr: fixed_opts for lr in DISTS})
+    fixed_index.close()
 
-    def one_step():
-        res = {}
-        for lr in DISTS:
-            _, _, t = index.query_timed(Q_dev[lr], out_val=out_v, out_idx=out_i, opts=opts)
-            res[lr] = t
-        return res
-
-    for w in range(args.warmup):
-        one_step()
-        log(f"bench: warmup {w + 1}/{args.warmup} done")
-
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    steps = []
-    for k in range(args.steps):
-        steps.append(one_step())
-        log(f"bench: step {k + 1}/{args.steps}: " + ", ".join(
-            f"{DIST_NAMES[lr]} {s['total_ms']:.2f} ms" for lr, s in steps[-1].items()))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-
-    total_queries = 3 * q * args.steps
-    value = total_queries / elapsed
-    per_dist = {}
-    long_bytes = long_ms = short_bytes = short_ms = 0.0
+    # --- autotuned configuration per distribution ----------------------------------
+    picks = {}
     for lr in DISTS:
-        lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs, opts.short_max)
-        ms = [s[lr]["total_ms"] for s in steps]
-        lms = [s[lr]["long_ms"] for s in steps]
-        sms = [s[lr]["short_ms"] for s in steps]
-        per_dist[DIST_NAMES[lr]] = {
-            "rmq_per_s": q / (np.mean(ms) / 1e3),
-            "ms_per_pass": float(np.mean(ms)),
-            "long_kernel_ms": float(np.mean(lms)),
-            "short_kernel_ms": float(np.mean(sms)),
-            "n_long": nl,
-            "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
-            "roofline_rmq_per_s": q / ((lb + sb) / (HBM_PEAK_GBS * 1e9)),
-        }
-        long_bytes += lb * args.steps
-        long_ms += float(np.sum(lms))
-        short_bytes += sb * args.steps
-        short_ms += float(np.sum(sms))
-    achieved = long_bytes / (long_ms / 1e3) / 1e9
-    workload = f"n={n} q={q} bs={bs} lr=-1,-2,-3 value+index"
-    traffic = load_traffic("k_query_long", workload)
+        sample = Q_dev[lr][: min(q, args.tune_sample)]
+        t0 = time.perf_counter()
+        picks[lr] = R.tune(A, sample)
+        p = picks[lr]
+        log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} short_max={p['opts'].short_max} "
+            f"group_lanes={p['opts'].group_lanes} long_lanes={p['opts'].long_lanes} "
+            f"({p['n_tried']} configs, {time.perf_counter() - t0:.1f} s, key {p['key']})")
+    indexes, build_ms = {}, {}
+    for lr in DISTS:
+        bs = picks[lr]["bs"]
+        if bs not in indexes:
+            t0 = time.perf_counter()
+            indexes[bs] = R.RMQIndex(A, bs)
+            torch.cuda.synchronize()
+            build_ms[bs] = (time.perf_counter() - t0) * 1e3
+    index_for = {lr: indexes[picks[lr]["bs"]] for lr in DISTS}
+    opts_for = {lr: picks[lr]["opts"] for lr in DISTS}
+    bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
+    steps_out, elapsed = timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, args.steps, args.warmup,
+                                   "tuned")
+    tuned = summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for)
+    log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
+        f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
+    for ix in indexes.values():
+        ix.close()
 
-    log(f"bench: long kernel {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+    workload = f"n={n} q={q} lr=-1,-2,-3 value+index tuned"
     cpu = None
     if not args.no_cpu_baseline:
         A_host = A.cpu().numpy()
@@ -212,29 +263,37 @@ def main() -> int:
 
     line = {
         "metric": "RMQ/s at n=1e8, q=2^26 over the three length distributions (uniform, lognormal 2^14, "
-                  "lognormal 2^6), bs=2^15",
-        "value": value,
+                  "lognormal 2^6)",
+        "value": tuned["value"],
         "unit": "RMQ/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": tuned["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: counter-based U[0,1) float32 array and host-generated queries, seed 1321",
-        "config": {"workload": "rtx_blocks plain (alg 5 + alg 105)", "n": n, "q": q, "bs": bs,
-                   "dists": [DIST_NAMES[lr] for lr in DISTS], "outputs": "value+index",
-                   "short_max": opts.short_max, "group_lanes": opts.group_lanes,
-                   "construction_ms": construction_ms},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_query_long"},
+        "config": {"workload": "rtx_blocks (alg 5 + alg 105, value+index), block size and thresholds autotuned "
+                               "per distribution",
+                   "n": n, "q": q, "dists": [DIST_NAMES[lr] for lr in DISTS],
+                   "bs": {DIST_NAMES[lr]: bs_for[lr] for lr in DISTS},
+                   "construction_ms": {str(k): v for k, v in build_ms.items()},
+                   "tune_sample": min(q, args.tune_sample)},
+        "roofline": {"bound": "hbm", "achieved": tuned["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": tuned["achieved"] / HBM_PEAK_GBS,
+                     "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"]},
         "cpu_baseline": cpu,
-        "per_dist": per_dist,
-        "short_kernel": {"achieved_GBps": short_bytes / max(short_ms, 1e-9) * 1e3 / 1e9},
+        "per_dist": tuned["per_dist"],
+        "per_kernel": tuned["per_kernel"],
+        f"fixed_bs_{args.bs}": {
+            "value": fixed["value"], "ms_per_step": fixed["ms_per_step"], "steps": args.fixed_steps,
+            "construction_ms": fixed_build_ms, "per_dist": fixed["per_dist"],
+            "roofline": {"kernel": fixed["dominant"], "achieved": fixed["achieved"],
+                         "frac": fixed["achieved"] / HBM_PEAK_GBS,
+                         "traffic": load_traffic(fixed["dominant"], f"n={n} q={q} bs={args.bs} lr=-1,-2,-3 value+index")}},
     }
-    index.close()
     print(json.dumps(line), flush=True)
     return 0
 

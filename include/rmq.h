@@ -57,7 +57,7 @@ typedef struct rmq_query_opts {
     int32_t short_max;   /* ranges of at most this many elements go to the group scan   */
     int32_t group_lanes; /* lanes per short query: 4, 8, 16 or 64                       */
     int32_t long_wgs;    /* persistent workgroups for the long-range kernel (0 = auto)  */
-    int32_t reserved;
+    int32_t long_lanes;  /* lanes per long query: 4, 8, 16 or 64                        */
 } rmq_query_opts_t;
 
 /* Per-kernel device time of one rmq_query_timed call, milliseconds. */
@@ -109,6 +109,28 @@ int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int64_t *nb,
 
 /* Default tuning for a handle (what a NULL opts means). */
 int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts);
+
+/* Autotuner pick: block size and query options for one (n, length
+ * distribution).  key is the distribution signature the cache is keyed by. */
+typedef struct rmq_tune_result {
+    int32_t bs;
+    rmq_query_opts_t opts;
+    float best_ms;      /* one query pass over the tuning sample */
+    int32_t n_tried;    /* configurations timed (0 when taken from the cache) */
+    int32_t from_cache;
+    char key[96];
+} rmq_tune_result_t;
+
+/* ---- autotuner (no counterpart in the reference, whose sweeps are shell
+ *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
+ *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^5 ..
+ *      2^16), then the short-range threshold and lanes per query at the best
+ *      block size, and returns the fastest configuration.  Results are cached
+ *      per (n, length-distribution signature) in the text file cache_path
+ *      (NULL: no cache).  d_val / d_idx are scratch outputs of q entries (one
+ *      may be NULL).  Synchronising; builds and frees its own handles. ---- */
+int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,
+             float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s, rmq_tune_result_t *out);
 
 /* ---- brute force on the device, algs 2/102 and the --check oracle:
  *      replaces kernel_rmq_basic / kernel_rmq_basic_idx / gpu_rmq_basic

@@ -40,12 +40,18 @@ class RMQError(RuntimeError):
 
 class QueryOpts(ctypes.Structure):
     """rmq_query_opts_t"""
-    _fields_ = [("short_max", _i32), ("group_lanes", _i32), ("long_wgs", _i32), ("reserved", _i32)]
+    _fields_ = [("short_max", _i32), ("group_lanes", _i32), ("long_wgs", _i32), ("long_lanes", _i32)]
 
 
 class QueryTimes(ctypes.Structure):
     """rmq_query_times_t"""
     _fields_ = [("short_ms", ctypes.c_float), ("long_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
+class TuneResult(ctypes.Structure):
+    """rmq_tune_result_t"""
+    _fields_ = [("bs", _i32), ("opts", QueryOpts), ("best_ms", ctypes.c_float), ("n_tried", _i32),
+                ("from_cache", _i32), ("key", ctypes.c_char * 96)]
 
 
 _LIB = None
@@ -64,6 +70,8 @@ def _declare(lib):
                                            ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
         "rmq_query_opts_default": (ctypes.c_int, [_vp, ctypes.POINTER(QueryOpts)]),
         "rmq_gpu_base": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
+        "rmq_tune": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, ctypes.c_char_p, _vp,
+                                    ctypes.POINTER(TuneResult)]),
         "rmq_gen_array": (ctypes.c_int, [_vp, _i64, _u64, _i32, _vp]),
         "rmq_gen_array_host": (ctypes.c_int, [_vp, _i64, _u64, _i32]),
         "rmq_gen_queries_host": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _i32]),

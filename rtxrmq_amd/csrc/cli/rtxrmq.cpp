@@ -53,7 +53,7 @@ struct Args {
     int n = 0, q = 0, lr = 0, alg = 0, bs = 1 << 15, nb = 0, reps = 10, dev = 0, nt = 1, seed = 0;
     bool check = false, save_time = false, save_power = false;
     std::string time_file, power_file;
-    int short_max = 0, group_lanes = 0;
+    int short_max = 0, group_lanes = 0, long_lanes = 0;
     int array_kind = RMQ_GEN_UNIFORM;
 };
 
@@ -80,6 +80,7 @@ void usage() {
             "  --save-power[=file]   sample GPU power during the query loop\n"
             "  --short-max <len>     ranges up to this length use the group scan (alg 5/105)\n"
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
+            "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n",
             kAlgName[0], kAlgName[1], kAlgName[2], kAlgName[5]);
 }
@@ -105,7 +106,7 @@ bool parse(int argc, char **argv, Args &a) {
     a.bs = std::min(a.n, 1 << 15);
     a.nb = a.n / a.bs;
     a.seed = (int)time(nullptr);
-    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY };
+    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG };
     static struct option opts[] = {
         {"bs", required_argument, nullptr, O_BS},         {"nb", required_argument, nullptr, O_NB},
         {"reps", required_argument, nullptr, O_REPS},     {"dev", required_argument, nullptr, O_DEV},
@@ -113,7 +114,8 @@ bool parse(int argc, char **argv, Args &a) {
         {"check", no_argument, nullptr, O_CHECK},         {"save-time", optional_argument, nullptr, O_TIME},
         {"save-power", optional_argument, nullptr, O_POWER},
         {"short-max", required_argument, nullptr, O_SHORT}, {"group-lanes", required_argument, nullptr, O_GROUP},
-        {"array", required_argument, nullptr, O_ARRAY},   {nullptr, 0, nullptr, 0}};
+        {"array", required_argument, nullptr, O_ARRAY},   {"long-lanes", required_argument, nullptr, O_LONG},
+        {nullptr, 0, nullptr, 0}};
     int opt, idx;
     optind = 5;
     while ((opt = getopt_long(argc, argv, "", opts, &idx)) != -1) {
@@ -129,6 +131,7 @@ bool parse(int argc, char **argv, Args &a) {
         case O_POWER: a.save_power = true; if (optarg) a.power_file = optarg; break;
         case O_SHORT: a.short_max = atoi(optarg); break;
         case O_GROUP: a.group_lanes = atoi(optarg); break;
+        case O_LONG: a.long_lanes = atoi(optarg); break;
         case O_ARRAY:
             if (!strcmp(optarg, "uniform")) a.array_kind = RMQ_GEN_UNIFORM;
             else if (!strcmp(optarg, "ties16")) a.array_kind = RMQ_GEN_TIES16;
@@ -320,7 +323,7 @@ int main(int argc, char **argv) {
         if (idx_out) hip_ok(hipMalloc(&d_idx, sizeof(int32_t) * (size_t)a.q), "hipMalloc(out)");
         else hip_ok(hipMalloc(&d_val, sizeof(float) * (size_t)a.q), "hipMalloc(out)");
         rmq_handle_t *h = nullptr;
-        rmq_query_opts_t opts{a.short_max, a.group_lanes, 0, 0};
+        rmq_query_opts_t opts{a.short_max, a.group_lanes, 0, a.long_lanes};
         const bool engine = a.alg == ALG_GPU_RTX_BLOCKS || a.alg == ALG_GPU_RTX_BLOCKS_IDX;
         if (engine) {
             printf("Building block index (bs=%i)..............", a.bs);

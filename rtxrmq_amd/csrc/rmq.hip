@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -226,125 +227,112 @@ __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
 }
 
-// Short ranges, G lanes per query, 64/G queries per round; every lane takes
-// part in the DPP reduction of every round.  Invalid queries get (NaN, -1).
-template <int G>
-__global__ __launch_bounds__(kWG) void k_query_short(const float *__restrict__ A, int n,
-                                                     const int2 *__restrict__ Q, int64_t q, int short_max,
-                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i) {
-    constexpr int QPR = kWave / G;  // queries per round
-    __shared__ int s_l[kWavesPerWG][kWave];
-    __shared__ int s_r[kWavesPerWG][kWave];
-    __shared__ int s_q[kWavesPerWG][kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int64_t n_tiles = (q + kWave - 1) / kWave;
-    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
-    const int g = lane / G;
-    const int gl = lane & (G - 1);
-    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, true, out_v, out_i);
-        if ((m.shorts >> lane) & 1) {
-            const uint32_t k = mask_rank(m.shorts);
-            s_l[wv][k] = m.l;
-            s_r[wv][k] = m.r;
-            s_q[wv][k] = lane;
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        const int n_short = __popcll(m.shorts);
-        for (int base = 0; base < n_short; base += QPR) {
-            const int s = base + g;
-            const bool has = s < n_short;
-            float bv = INFINITY;
-            int bi = INT_MAX;
-            int ql = 0;
-            if (has) {
-                ql = s_l[wv][s];
-                scan_range<G, 1>(A, ql, s_r[wv][s], gl, bv, bi);
+// One query [l, r] answered by the G lanes of a group (gl = lane in group).
+// Short class: one direct scan.  Long class: lB == rB is one in-block scan
+// (src/rtx_kernels.cu:116-135); otherwise the left edge block from l and the
+// right edge block up to r (:151-177) are scanned and, when whole blocks lie
+// between, the sparse table over block minima (:138-149) gives their leftmost
+// minimum in O(1).  The group's result is combined with the table's with the
+// index as tie-break, so the leftmost minimum wins.
+template <int G, int U, bool LONG>
+__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, int nb,
+                                               const Pair *__restrict__ st, int l, int r, int gl, float &bv,
+                                               int &bi, float &mv, int &mi) {
+    if constexpr (!LONG) {
+        scan_range<G, U>(A, l, r, gl, bv, bi);
+    } else {
+        const int lB = l / bs, rB = r / bs;
+        if (lB == rB) {
+            scan_range<G, U>(A, l, r, gl, bv, bi);
+        } else {
+            if (rB - lB > 1) {
+                const int a = lB + 1, c = rB - lB - 1;
+                const int lev = 31 - __clz(c);
+                const Pair *row = st + (int64_t)lev * nb;
+                const Pair x = row[a];
+                const Pair y = row[a + c - (1 << lev)];
+                const Pair m = (y.v < x.v) ? y : x;
+                mv = m.v;
+                mi = m.i;
             }
-            group_reduce<G>(bv, bi);
-            if (has && gl == 0) {
-                if (bv == INFINITY) bi = ql;
-                write_result(out_v, out_i, t * kWave + s_q[wv][s], bv, bi);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-}
-
-// Short ranges with the whole wave per query (group of 64 lanes).
-__global__ __launch_bounds__(kWG) void k_query_short64(const float *__restrict__ A, int n,
-                                                       const int2 *__restrict__ Q, int64_t q, int short_max,
-                                                       float *__restrict__ out_v, int32_t *__restrict__ out_i) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int64_t n_tiles = (q + kWave - 1) / kWave;
-    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
-    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, true, out_v, out_i);
-        for (uint64_t sm = m.shorts; sm; sm &= sm - 1) {
-            const int src = __ffsll((unsigned long long)sm) - 1;
-            const int ql = __builtin_amdgcn_readlane(m.l, src);
-            const int qr = __builtin_amdgcn_readlane(m.r, src);
-            float bv = INFINITY;
-            int bi = INT_MAX;
-            scan_range<kWave, 2>(A, ql, qr, lane, bv, bi);
-            wave_reduce(bv, bi);
-            if (lane == 0) {
-                if (bv == INFINITY) bi = ql;
-                write_result(out_v, out_i, t * kWave + src, bv, bi);
-            }
+            scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);
+            scan_range<G, U>(A, rB * bs, r, gl, bv, bi);
         }
     }
 }
 
-// Long ranges, one whole wave per query.  lB == rB: one in-block scan
-// (src/rtx_kernels.cu:116-135).  Otherwise the left edge block from l, the
-// right edge block up to r (:151-177) and, when whole blocks lie between,
-// the sparse table over block minima (:138-149), combined with the index as
-// tie-break so the leftmost minimum wins.
-__global__ __launch_bounds__(kWG) void k_query_long(const float *__restrict__ A, int n, int bs, int nb,
-                                                    const Pair *__restrict__ st, const int2 *__restrict__ Q,
-                                                    int64_t q, int short_max, float *__restrict__ out_v,
-                                                    int32_t *__restrict__ out_i) {
+// Tile kernel: each wave takes tiles of 64 consecutive queries and answers
+// the ones of its class (LONG = more than short_max elements), G lanes per
+// query.  G == 64: the whole wave per query, queries taken from the ballot
+// mask in lane order.  G < 64: the class's queries are staged in LDS and
+// answered 64/G per round, every lane taking part in each round's DPP
+// reduction.  The short-class kernel also writes (NaN, -1) for invalid
+// queries.  Persistent grid, static stride over tiles.
+template <int G, bool LONG>
+__global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A, int n, int bs, int nb,
+                                                     const Pair *__restrict__ st, const int2 *__restrict__ Q,
+                                                     int64_t q, int short_max, float *__restrict__ out_v,
+                                                     int32_t *__restrict__ out_i) {
+    constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
-    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, false, out_v, out_i);
-        for (uint64_t lm = m.longs; lm; lm &= lm - 1) {
-            const int src = __ffsll((unsigned long long)lm) - 1;
-            const int l = __builtin_amdgcn_readlane(m.l, src);
-            const int r = __builtin_amdgcn_readlane(m.r, src);
-            const int lB = l / bs, rB = r / bs;
-            float bv = INFINITY;
-            int bi = INT_MAX;
-            float mv = INFINITY;
-            int mi = INT_MAX;
-            if (lB == rB) {
-                scan_range<kWave, kLongUnroll>(A, l, r, lane, bv, bi);
-            } else {
-                if (rB - lB > 1) {
-                    const int a = lB + 1, c = rB - lB - 1;
-                    const int lev = 31 - __clz(c);
-                    const Pair *row = st + (int64_t)lev * nb;
-                    const Pair x = row[a];
-                    const Pair y = row[a + c - (1 << lev)];
-                    const Pair mm = (y.v < x.v) ? y : x;
-                    mv = mm.v;
-                    mi = mm.i;
+    if constexpr (G == kWave) {
+        for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+            const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
+            for (uint64_t mk = LONG ? m.longs : m.shorts; mk; mk &= mk - 1) {
+                const int src = __ffsll((unsigned long long)mk) - 1;
+                const int l = __builtin_amdgcn_readlane(m.l, src);
+                const int r = __builtin_amdgcn_readlane(m.r, src);
+                float bv = INFINITY, mv = INFINITY;
+                int bi = INT_MAX, mi = INT_MAX;
+                answer_partial<kWave, U, LONG>(A, bs, nb, st, l, r, lane, bv, bi, mv, mi);
+                wave_reduce(bv, bi);
+                take_if_better(mv, mi, bv, bi);
+                if (lane == 0) {
+                    if (bv == INFINITY) bi = l;
+                    write_result(out_v, out_i, t * kWave + src, bv, bi);
                 }
-                scan_range<kWave, kLongUnroll>(A, l, lB * bs + bs - 1, lane, bv, bi);
-                scan_range<kWave, kLongUnroll>(A, rB * bs, r, lane, bv, bi);
             }
-            wave_reduce(bv, bi);
-            take_if_better(mv, mi, bv, bi);
-            if (lane == 0) {
-                if (bv == INFINITY) bi = l;
-                write_result(out_v, out_i, t * kWave + src, bv, bi);
+        }
+    } else {
+        constexpr int QPR = kWave / G;  // queries per round
+        __shared__ int s_l[kWavesPerWG][kWave];
+        __shared__ int s_r[kWavesPerWG][kWave];
+        __shared__ int s_q[kWavesPerWG][kWave];
+        const int g = lane / G;
+        const int gl = lane & (G - 1);
+        for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+            const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
+            const uint64_t mk = LONG ? m.longs : m.shorts;
+            if ((mk >> lane) & 1) {
+                const uint32_t k = mask_rank(mk);
+                s_l[wv][k] = m.l;
+                s_r[wv][k] = m.r;
+                s_q[wv][k] = lane;
             }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const int count = __popcll(mk);
+            for (int base = 0; base < count; base += QPR) {
+                const int s = base + g;
+                const bool has = s < count;
+                float bv = INFINITY, mv = INFINITY;
+                int bi = INT_MAX, mi = INT_MAX;
+                int ql = 0;
+                if (has) {
+                    ql = s_l[wv][s];
+                    answer_partial<G, U, LONG>(A, bs, nb, st, ql, s_r[wv][s], gl, bv, bi, mv, mi);
+                }
+                group_reduce<G>(bv, bi);
+                take_if_better(mv, mi, bv, bi);
+                if (has && gl == 0) {
+                    if (bv == INFINITY) bi = ql;
+                    write_result(out_v, out_i, t * kWave + s_q[wv][s], bv, bi);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -371,6 +359,11 @@ __global__ void k_gpu_base(const float *__restrict__ A, const int2 *__restrict__
         }
     }
     write_result(out_v, out_i, tid, m, mi);
+}
+
+__global__ void k_gather_stride(const int2 *__restrict__ Q, int64_t step, int64_t m, int2 *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) out[k] = Q[k * step];
 }
 
 __global__ void k_count_bad(const int2 *__restrict__ Q, int64_t q, int64_t n, unsigned long long *bad) {
@@ -473,7 +466,7 @@ void fill_default_opts(const rmq_handle *h, rmq_query_opts_t *o) {
     o->short_max = kDefaultShortMax;
     o->group_lanes = kDefaultGroupLanes;
     o->long_wgs = 0;
-    o->reserved = 0;
+    o->long_lanes = kWave;
 }
 
 int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts_t *o) {
@@ -481,43 +474,49 @@ int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts
     if (in) {
         if (in->short_max < 0 || in->long_wgs < 0) return RMQ_ERR_INVALID_ARG;
         if (in->short_max > 0) o->short_max = in->short_max;
+        auto lanes_ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 64; };
         if (in->group_lanes != 0) {
-            if (in->group_lanes != 4 && in->group_lanes != 8 && in->group_lanes != 16 && in->group_lanes != 64)
-                return RMQ_ERR_INVALID_ARG;
+            if (!lanes_ok(in->group_lanes)) return RMQ_ERR_INVALID_ARG;
             o->group_lanes = in->group_lanes;
+        }
+        if (in->long_lanes != 0) {
+            if (!lanes_ok(in->long_lanes)) return RMQ_ERR_INVALID_ARG;
+            o->long_lanes = in->long_lanes;
         }
         if (in->long_wgs > 0) o->long_wgs = in->long_wgs;
     }
     return RMQ_OK;
 }
 
-int short_grid(int64_t q) {
+int tiles_grid(int64_t q, int per_cu) {
     const int64_t tiles = (q + kWave - 1) / kWave;
     const int64_t want = (tiles + kWavesPerWG - 1) / kWavesPerWG;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)device_cus() * 16));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)device_cus() * per_cu));
 }
 
-int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
-                 hipStream_t s) {
-    const int grid = short_grid(q);
-    const int n = (int)h->n;
-    switch (o.group_lanes) {
-    case 4: k_query_short<4><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
-    case 8: k_query_short<8><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
-    case 16: k_query_short<16><<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
-    default: k_query_short64<<<grid, kWG, 0, s>>>(h->A, n, Q, q, o.short_max, val, idx); break;
+template <bool LONG>
+int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int grid, int short_max, float *val,
+                 int32_t *idx, hipStream_t s) {
+    const int n = (int)h->n, bs = h->bs, nb = (int)h->nb;
+    switch (lanes) {
+    case 4: k_query_tiles<4, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
+    case 8: k_query_tiles<8, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
+    case 16: k_query_tiles<16, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
+    default: k_query_tiles<64, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
     }
     return hip_rc(hipGetLastError());
 }
 
+int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
+                 hipStream_t s) {
+    return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), o.short_max, val, idx, s);
+}
+
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                 hipStream_t s) {
-    const int64_t tiles = (q + kWave - 1) / kWave;
-    const int64_t want = (tiles + kWavesPerWG - 1) / kWavesPerWG;
-    int grid = o.long_wgs > 0 ? o.long_wgs : device_cus() * 8;
-    grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, want));
-    k_query_long<<<grid, kWG, 0, s>>>(h->A, (int)h->n, h->bs, (int)h->nb, h->st, Q, q, o.short_max, val, idx);
-    return hip_rc(hipGetLastError());
+    int grid = tiles_grid(q, 8);
+    if (o.long_wgs > 0) grid = (int)std::min<int64_t>(o.long_wgs, grid);
+    return launch_tiles<true>(h, Q, q, o.long_lanes, grid, o.short_max, val, idx, s);
 }
 
 int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
@@ -765,6 +764,164 @@ RMQ_API int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, u
         pool.emplace_back(work, lo, hi);
     }
     for (auto &th : pool) th.join();
+    return RMQ_OK;
+}
+
+// ===========================================================================
+// autotuner
+// ===========================================================================
+namespace {
+
+// Length-distribution signature of a query batch: log2 n and the 10th, 50th
+// and 90th percentiles of log2(range length), in half steps, over up to 2^16
+// evenly spaced queries.
+int tune_signature(const int32_t *d_Q, int64_t q, int64_t n, hipStream_t s, char *key, size_t key_len) {
+    const int64_t m = std::min<int64_t>(q, 1 << 16);
+    const int64_t step = q / m;
+    std::vector<int32_t> sample(2 * (size_t)m);
+    int2 *d_s = nullptr;
+    RMQ_HIP(hipMalloc(&d_s, sizeof(int2) * (size_t)m));
+    k_gather_stride<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(reinterpret_cast<const int2 *>(d_Q), step, m, d_s);
+    int rc = hip_rc(hipGetLastError());
+    if (rc == RMQ_OK) rc = hip_rc(hipMemcpyAsync(sample.data(), d_s, sizeof(int2) * (size_t)m, hipMemcpyDeviceToHost, s));
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    (void)hipFree(d_s);
+    if (rc != RMQ_OK) return rc;
+    std::vector<double> lg((size_t)m);
+    for (int64_t k = 0; k < m; ++k) lg[k] = std::log2(std::max(1.0, (double)sample[2 * k + 1] - sample[2 * k] + 1));
+    std::sort(lg.begin(), lg.end());
+    auto pct = [&](double p) { return std::round(2.0 * lg[(size_t)std::min<double>(m - 1, p * m)]) / 2.0; };
+    snprintf(key, key_len, "n2^%.1f/p10=%.1f/p50=%.1f/p90=%.1f", std::round(2.0 * std::log2((double)n)) / 2.0,
+             pct(0.10), pct(0.50), pct(0.90));
+    return RMQ_OK;
+}
+
+bool cache_lookup(const char *path, const char *key, rmq_tune_result_t *out) {
+    FILE *f = path ? fopen(path, "r") : nullptr;
+    if (!f) return false;
+    char line[512], k[256];
+    bool found = false;
+    while (fgets(line, sizeof(line), f)) {
+        rmq_tune_result_t r{};
+        if (sscanf(line, "%255s %d %d %d %d %f", k, &r.bs, &r.opts.short_max, &r.opts.group_lanes,
+                   &r.opts.long_lanes, &r.best_ms) == 6 &&
+            strcmp(k, key) == 0) {
+            snprintf(r.key, sizeof(r.key), "%s", key);
+            *out = r;  // last entry for a key wins
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+void cache_store(const char *path, const char *key, const rmq_tune_result_t &r) {
+    FILE *f = path ? fopen(path, "a") : nullptr;
+    if (!f) return;
+    fprintf(f, "%s %d %d %d %d %.6f\n", key, r.bs, r.opts.short_max, r.opts.group_lanes, r.opts.long_lanes, r.best_ms);
+    fclose(f);
+}
+
+// Best of `reps` timed passes (after one warm-up pass), milliseconds.
+int time_pass(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+              const rmq_query_opts_t &o, hipStream_t s, int reps, float *best) {
+    rmq_query_times_t t;
+    int rc = query_impl(h, d_Q, q, d_val, d_idx, &o, s, &t);
+    *best = 1e30f;
+    for (int k = 0; rc == RMQ_OK && k < reps; ++k) {
+        rc = query_impl(h, d_Q, q, d_val, d_idx, &o, s, &t);
+        if (rc == RMQ_OK) *best = std::min(*best, t.total_ms);
+    }
+    return rc;
+}
+
+}  // namespace
+
+RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list,
+                     int32_t n_bs, float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s,
+                     rmq_tune_result_t *out) {
+    if (!out || !d_A || n <= 0 || n > INT32_MAX || q <= 0 || !d_Q || (!d_val && !d_idx) || n_bs < 0 ||
+        (n_bs > 0 && !bs_list))
+        return RMQ_ERR_INVALID_ARG;
+    *out = rmq_tune_result_t{};
+    char key[160];
+    int rc = tune_signature(d_Q, q, n, s, key, sizeof(key));
+    if (rc != RMQ_OK) return rc;
+    if (cache_lookup(cache_path, key, out)) {
+        out->from_cache = 1;
+        out->n_tried = 0;
+        return RMQ_OK;
+    }
+    std::vector<int32_t> cands;
+    if (n_bs > 0) {
+        cands.assign(bs_list, bs_list + n_bs);
+    } else {
+        for (int b = 5; b <= 16; ++b) cands.push_back(1 << b);
+    }
+    auto base_opts = [](int32_t bs) {
+        rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, bs <= 2048 ? 16 : kWave};
+        return o;
+    };
+    float best = 1e30f;
+    int32_t best_bs = 0;
+    int tried = 0;
+    // stage 1: block size with the default thresholds
+    for (int32_t bs : cands) {
+        if (bs <= 0 || bs > n) continue;
+        rmq_handle_t *h = nullptr;
+        if ((rc = rmq_build(d_A, n, bs, 0, s, &h)) != RMQ_OK) return rc;
+        float ms = 0.f;
+        rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 2, &ms);
+        rmq_free(h);
+        if (rc != RMQ_OK) return rc;
+        ++tried;
+        if (ms < best) {
+            best = ms;
+            best_bs = bs;
+        }
+    }
+    if (best_bs == 0) return RMQ_ERR_INVALID_ARG;
+    rmq_handle_t *h = nullptr;
+    if ((rc = rmq_build(d_A, n, best_bs, 0, s, &h)) != RMQ_OK) return rc;
+    rmq_query_opts_t best_o = base_opts(best_bs);
+    // stage 2: short-range threshold and lanes per short query
+    for (int32_t sm : {32, 128, 512, 1024, 4096}) {
+        for (int32_t g : {4, 8, 16}) {
+            rmq_query_opts_t o = best_o;
+            o.short_max = sm;
+            o.group_lanes = g;
+            float ms = 0.f;
+            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 2, &ms)) != RMQ_OK) break;
+            ++tried;
+            if (ms < best) {
+                best = ms;
+                best_o = o;
+            }
+        }
+        if (rc != RMQ_OK) break;
+    }
+    // stage 3: lanes per long query
+    for (int32_t g : {4, 8, 16, 64}) {
+        if (rc != RMQ_OK) break;
+        rmq_query_opts_t o = best_o;
+        o.long_lanes = g;
+        float ms = 0.f;
+        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 2, &ms)) != RMQ_OK) break;
+        ++tried;
+        if (ms < best) {
+            best = ms;
+            best_o = o;
+        }
+    }
+    rmq_free(h);
+    if (rc != RMQ_OK) return rc;
+    out->bs = best_bs;
+    out->opts = best_o;
+    out->best_ms = best;
+    out->n_tried = tried;
+    out->from_cache = 0;
+    snprintf(out->key, sizeof(out->key), "%s", key);
+    cache_store(cache_path, key, *out);
     return RMQ_OK;
 }
 

@@ -59,8 +59,8 @@ def _dev_queries(t):
     return t
 
 
-def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0) -> N.QueryOpts:
-    return N.QueryOpts(short_max, group_lanes, long_wgs, 0)
+def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0, long_lanes: int = 0) -> N.QueryOpts:
+    return N.QueryOpts(short_max, group_lanes, long_wgs, long_lanes)
 
 
 class RMQIndex:
@@ -166,6 +166,28 @@ def check_queries(Q, n: int, stream=None) -> int:
     if rc not in (N.RMQ_OK, N.RMQ_ERR_BAD_QUERY):
         N.check(rc, "rmq_query_check")
     return bad.value
+
+
+def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, stream=None) -> dict:
+    """Autotuner pick (rmq_tune) for this array and a sample of its queries.
+
+    Returns {"bs", "opts" (QueryOpts), "best_ms", "n_tried", "from_cache", "key"}.
+    """
+    torch = _torch()
+    A = _dev_f32(A, "A")
+    Q_sample = _dev_queries(Q_sample)
+    q = Q_sample.numel() // 2
+    scratch = torch.empty(q, dtype=torch.int32, device=A.device)
+    arr = (ctypes.c_int32 * len(bs_list))(*bs_list) if bs_list else None
+    res = N.TuneResult()
+    rc = N.lib().rmq_tune(A.data_ptr(), A.numel(), Q_sample.data_ptr(), q, arr, len(bs_list) if bs_list else 0,
+                          None, scratch.data_ptr(), cache_path.encode() if cache_path else None,
+                          _stream_ptr(stream), ctypes.byref(res))
+    N.check(rc, "rmq_tune")
+    o = res.opts
+    return {"bs": res.bs, "opts": N.QueryOpts(o.short_max, o.group_lanes, o.long_wgs, o.long_lanes),
+            "best_ms": res.best_ms, "n_tried": res.n_tried, "from_cache": bool(res.from_cache),
+            "key": res.key.decode()}
 
 
 def gen_array(n: int, seed: int, kind: str = "uniform", device="cuda", stream=None):

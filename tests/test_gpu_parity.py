@@ -64,12 +64,32 @@ def mixed_case():
 
 
 @pytest.mark.parametrize("group_lanes", [4, 8, 16, 64])
+@pytest.mark.parametrize("long_lanes", [4, 16, 64])
 @pytest.mark.parametrize("short_max", [1, 7, 64, 1024, 1 << 30])
-def test_tuning_knobs_do_not_change_answers(group_lanes, short_max, mixed_case, gpu):
+def test_tuning_knobs_do_not_change_answers(group_lanes, long_lanes, short_max, mixed_case, gpu):
     A, Q, ev, ei = mixed_case
-    opts = R.query_opts(short_max=short_max, group_lanes=group_lanes)
-    v, i = run_engine(A, Q, 1000, gpu, opts=opts)
-    assert_exact(v, i, ev, ei, f"G={group_lanes} short_max={short_max}")
+    opts = R.query_opts(short_max=short_max, group_lanes=group_lanes, long_lanes=long_lanes)
+    for bs in (37, 1000):
+        v, i = run_engine(A, Q, bs, gpu, opts=opts)
+        assert_exact(v, i, ev, ei, f"bs={bs} G={group_lanes} GL={long_lanes} short_max={short_max}")
+
+
+def test_autotuner_pick_and_cache(tmp_path, gpu):
+    n = 1 << 22
+    A_dev = R.gen_array(n, 5, device=gpu)
+    Q = R.gen_queries(1 << 16, n, -2, 5)
+    Q_dev = _dev(Q, gpu)
+    cache = str(tmp_path / "tune.txt")
+    r1 = R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)
+    assert r1["bs"] in (256, 1024, 4096) and r1["n_tried"] > 3 and not r1["from_cache"]
+    assert r1["opts"].group_lanes in (4, 8, 16, 64) and r1["opts"].long_lanes in (4, 8, 16, 64)
+    r2 = R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)
+    assert r2["from_cache"] and r2["bs"] == r1["bs"] and r2["key"] == r1["key"]
+    with R.RMQIndex(A_dev, r1["bs"]) as ix:
+        v, i = ix.query(Q_dev, opts=r1["opts"])
+        torch.cuda.synchronize()
+    ev, ei = O.rmq_brute(A_dev.cpu().numpy(), Q)
+    assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, "tuned config")
 
 
 def test_config1_plain_small(gpu):
