@@ -177,6 +177,92 @@ def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for) -> dict:
             "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
 
 
+def run_batched(R, torch, dev, args) -> dict:
+    """Config 4: m arrays with n_i log-uniform in [2^10, 2^22], q_i = n_i queries
+    per array (1:1:1 uniform / lognormal 2^14-scaled / lognormal 2^6-scaled
+    lengths), int32 leftmost indices; one persistent plan launch (HIP graph)
+    against the per-array launch loop on the same handles and buffers."""
+    import ctypes
+    from rtxrmq_amd import _native as N
+
+    rng = np.random.default_rng(args.seed)
+    m = args.batch_arrays
+    ns = np.floor(np.exp2(rng.uniform(10, 22, m))).astype(np.int64)
+    a_off = np.zeros(m + 1, np.int64)
+    a_off[1:] = np.cumsum((ns + 63) // 64 * 64)  # 256-byte aligned array starts
+    q_off = np.zeros(m + 1, np.int64)
+    q_off[1:] = np.cumsum(ns)
+    total_q = int(q_off[-1])
+    log(f"batched: {m} arrays, {int(ns.sum())} elements, {total_q} queries")
+    A_all = torch.empty(int(a_off[-1]), dtype=torch.float32, device=dev)
+    N.check(N.lib().rmq_gen_array(A_all.data_ptr(), A_all.numel(), args.seed, 0, None), "rmq_gen_array")
+    Q_all = torch.empty((total_q, 2), dtype=torch.int32, device=dev)
+    out_plan = torch.empty(total_q, dtype=torch.int32, device=dev)
+    out_loop = torch.empty(total_q, dtype=torch.int32, device=dev)
+    A_views, Q_views = [], []
+    for i in range(m):
+        n_i = int(ns[i])
+        A_views.append(A_all[a_off[i]:a_off[i] + n_i])
+        Qi = Q_all[q_off[i]:q_off[i + 1]]
+        Q_views.append(Qi)
+        parts = np.array_split(np.arange(n_i), 3)
+        for lr, part in zip((-1, -7, -8), parts):
+            if len(part):
+                sub = Qi[int(part[0]):int(part[-1]) + 1]
+                N.check(N.lib().rmq_gen_queries(sub.data_ptr(), len(part), n_i, lr, args.seed * 7919 + i, None),
+                        "rmq_gen_queries")
+    torch.cuda.synchronize()
+    big = int(np.argmax(ns))
+    pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096])
+    opts, bs = pick["opts"], pick["bs"]
+    log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} short_max={opts.short_max} "
+        f"group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
+    t0 = time.perf_counter()
+    indexes = [R.RMQIndex(A, bs) for A in A_views]
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    outs_p = [out_plan[q_off[i]:q_off[i + 1]] for i in range(m)]
+    outs_l = [out_loop[q_off[i]:q_off[i + 1]] for i in range(m)]
+    t0 = time.perf_counter()
+    plan = R.Plan(indexes, Q_views, out_idx=outs_p, opts=opts)
+    plan_s = time.perf_counter() - t0
+    P = ctypes.c_void_p * m
+    hs = P(*[ix.handle for ix in indexes])
+    qp = P(*[Q.data_ptr() for Q in Q_views])
+    qs = (ctypes.c_int64 * m)(*[int(x) for x in ns])
+    ip = P(*[t.data_ptr() for t in outs_l])
+
+    def loop():
+        N.check(N.lib().rmq_query_loop(hs, m, qp, qs, None, ip, ctypes.byref(opts), None), "rmq_query_loop")
+
+    def timed(fn, steps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    t_graph = timed(plan.graph_launch, args.batch_steps)
+    t_eager = timed(plan.launch, args.batch_steps)
+    t_loop = timed(loop, args.batch_steps)
+    identical = bool(torch.equal(out_plan, out_loop))
+    log(f"batched: graph {total_q / t_graph:.4e} RMQ/s ({t_graph * 1e3:.1f} ms), eager plan "
+        f"{total_q / t_eager:.4e}, per-array loop {total_q / t_loop:.4e} ({t_loop * 1e3:.1f} ms); "
+        f"identical={identical}")
+    info = plan.info()
+    plan.close()
+    for ix in indexes:
+        ix.close()
+    return {"arrays": m, "elements": int(ns.sum()), "queries": total_q, "bs": bs,
+            "opts": {"short_max": opts.short_max, "group_lanes": opts.group_lanes, "long_lanes": opts.long_lanes},
+            "work_items": info["items"], "rmq_per_s_graph": total_q / t_graph, "rmq_per_s_plan": total_q / t_eager,
+            "rmq_per_s_loop": total_q / t_loop, "speedup_vs_loop": t_loop / t_graph,
+            "ms_graph": t_graph * 1e3, "ms_loop": t_loop * 1e3, "build_s": build_s, "plan_create_s": plan_s,
+            "steps": args.batch_steps, "identical_to_loop": identical}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -191,6 +277,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1321)
     ap.add_argument("--cpu-seconds", type=float, default=1.5, help="wall seconds of HRMQ per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -256,6 +345,8 @@ def main() -> int:
         ix.close()
 
     workload = f"n={n} q={q} lr=-1,-2,-3 value+index tuned"
+    del out_v, out_i
+    batched = None if args.no_batched else run_batched(R, torch, dev, args)
     cpu = None
     if not args.no_cpu_baseline:
         A_host = A.cpu().numpy()
@@ -285,6 +376,7 @@ def main() -> int:
                      "frac": tuned["achieved"] / HBM_PEAK_GBS,
                      "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"]},
         "cpu_baseline": cpu,
+        "batched": batched,
         "per_dist": tuned["per_dist"],
         "per_kernel": tuned["per_kernel"],
         f"fixed_bs_{args.bs}": {

@@ -123,7 +123,7 @@ typedef struct rmq_tune_result {
 
 /* ---- autotuner (no counterpart in the reference, whose sweeps are shell
  *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
- *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^5 ..
+ *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^3 ..
  *      2^16), then the short-range threshold and lanes per query at the best
  *      block size, and returns the fastest configuration.  Results are cached
  *      per (n, length-distribution signature) in the text file cache_path
@@ -131,6 +131,33 @@ typedef struct rmq_tune_result {
  *      may be NULL).  Synchronising; builds and frees its own handles. ---- */
 int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,
              float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s, rmq_tune_result_t *out);
+
+/* ---- multi-array plans: many arrays (the batched mode) or many requests (the
+ *      serving loop) answered by ONE persistent launch.  No counterpart in the
+ *      reference, which answers one array per run (src/main.cu:51-171).
+ *      A plan holds, on the device, one descriptor per handle and a work list
+ *      of chunks of at most item_queries (0: 4096) queries; array i's d_Qs[i]
+ *      holds qs[i] pairs {l, r} (0 <= l <= r < n_i) and its answers go to
+ *      d_vals[i] and/or d_idxs[i] (per-array NULL allowed; d_vals or d_idxs
+ *      may be NULL as a whole).  opts applies to every array (NULL =
+ *      defaults).  The pointers are captured at creation: the buffers must
+ *      outlive the plan, their contents may change between launches.
+ *      Creation is synchronising; launches are stream-ordered. ---- */
+typedef struct rmq_plan rmq_plan_t;
+int rmq_plan_create(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs, const int64_t *qs,
+                    float *const *d_vals, int32_t *const *d_idxs, const rmq_query_opts_t *opts, int32_t item_queries,
+                    hipStream_t s, rmq_plan_t **out);
+/* Enqueues the plan: a one-thread kernel resets the device queue head, then
+ * the persistent kernel drains the work list (no memset; graph-capturable). */
+int rmq_plan_launch(const rmq_plan_t *p, hipStream_t s);
+/* Same work through a HIP graph captured on the first call and replayed after. */
+int rmq_plan_graph_launch(rmq_plan_t *p, hipStream_t s);
+int rmq_plan_info(const rmq_plan_t *p, int32_t *n_arrays, int32_t *n_items, int64_t *n_queries);
+void rmq_plan_free(rmq_plan_t *p);
+/* The per-array launch loop the plan is measured against: rmq_query_ex on
+ * each array in turn, same arguments as rmq_plan_create. */
+int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs, const int64_t *qs,
+                   float *const *d_vals, int32_t *const *d_idxs, const rmq_query_opts_t *opts, hipStream_t s);
 
 /* ---- brute force on the device, algs 2/102 and the --check oracle:
  *      replaces kernel_rmq_basic / kernel_rmq_basic_idx / gpu_rmq_basic
@@ -146,10 +173,15 @@ int rmq_gpu_base(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, flo
  *      lr > 0: fixed length lr; -1 uniform length in [1, n-l]; -2 / -3
  *      lognormal round(exp(N(ln 2^14 | ln 2^6, 1))) clamped to [1, n-l];
  *      -4 / -5 lognormal with the reference's scales n/2^8, n/2^15 and
- *      sigma 0.3 (src/rand.cuh:194-198), clamped; -6 r = n-1. ---- */
+ *      sigma 0.3 (src/rand.cuh:194-198), clamped; -6 r = n-1; -7 / -8 the
+ *      -2 / -3 lognormals with medians scaled to n (n^(14/log2 1e8),
+ *      n^(6/log2 1e8)).  rmq_gen_queries runs the same stream on the device;
+ *      its lognormal lengths can differ from the host's where the device and
+ *      host libm round differently (integer-only codes are identical). ---- */
 int rmq_gen_array(float *d_A, int64_t n, uint64_t seed, int32_t kind, hipStream_t s);
 int rmq_gen_array_host(float *A, int64_t n, uint64_t seed, int32_t kind);
 int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int32_t nthreads);
+int rmq_gen_queries(int32_t *d_Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, hipStream_t s);
 
 /* Static description of a return code. */
 const char *rmq_strerror(int code);

@@ -326,6 +326,9 @@ ORC_API void orc_fast_free(void *h) {
  *   lr = -4  l uniform, length round(exp(N(ln max(1,n/2^8), 0.3))), clamped
  *   lr = -5  l uniform, length round(exp(N(ln max(1,n/2^15), 0.3))), clamped
  *   lr = -6  l uniform, r = n-1
+ *   lr = -7  l uniform, length round(exp(N(14/log2(1e8) * ln n, 1))), clamped
+ *   lr = -8  l uniform, length round(exp(N(6/log2(1e8) * ln n, 1))), clamped
+ *            (the -2 / -3 medians 2^14, 2^6 at n=1e8, scaled to n)
  */
 #define PHI 0x9E3779B97F4A7C15ULL
 
@@ -371,13 +374,15 @@ static double lognormal_len(uint64_t k3, uint64_t k4, uint64_t i, double mu, dou
 }
 
 ORC_API int orc_gen_queries(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int nt) {
-    if (n <= 0 || lr == 0 || lr > n || lr < -6) return -1;
+    if (n <= 0 || lr == 0 || lr > n || lr < -8) return -1;
     uint64_t k1 = rng_key(seed, 1), k2 = rng_key(seed, 2), k3 = rng_key(seed, 3), k4 = rng_key(seed, 4);
     double mu = 0.0, sigma = 1.0;
     if (lr == -2) mu = 14.0 * 0.69314718055994530942;
     if (lr == -3) mu = 6.0 * 0.69314718055994530942;
     if (lr == -4) { int64_t s = n / 256; mu = log((double)(s > 1 ? s : 1)); sigma = 0.3; }
     if (lr == -5) { int64_t s = n / 32768; mu = log((double)(s > 1 ? s : 1)); sigma = 0.3; }
+    if (lr == -7) mu = 14.0 / log2(1e8) * log((double)n);
+    if (lr == -8) mu = 6.0 / log2(1e8) * log((double)n);
 #ifdef _OPENMP
     if (nt > 0) omp_set_num_threads(nt);
 #pragma omp parallel for schedule(static)

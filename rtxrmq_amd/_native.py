@@ -75,12 +75,18 @@ def _declare(lib):
         "rmq_gen_array": (ctypes.c_int, [_vp, _i64, _u64, _i32, _vp]),
         "rmq_gen_array_host": (ctypes.c_int, [_vp, _i64, _u64, _i32]),
         "rmq_gen_queries_host": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _i32]),
+        "rmq_gen_queries": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _vp]),
+        "rmq_plan_create": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _i32, _vp,
+                                           ctypes.POINTER(_vp)]),
+        "rmq_plan_launch": (ctypes.c_int, [_vp, _vp]),
+        "rmq_plan_graph_launch": (ctypes.c_int, [_vp, _vp]),
+        "rmq_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
+        "rmq_plan_free": (None, [_vp]),
+        "rmq_query_loop": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _vp]),
         "rmq_strerror": (ctypes.c_char_p, [ctypes.c_int]),
         "rmq_version": (ctypes.c_char_p, []),
     }
-    optional = {
-        "rmq_batch_build": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, ctypes.POINTER(_vp)]),
-    }
+    optional = {}
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res

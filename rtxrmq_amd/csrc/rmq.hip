@@ -25,122 +25,9 @@
 #include <thread>
 #include <vector>
 
-#include "rmq.h"
+#include "rmq_common.h"
 
-#define RMQ_API extern "C" __attribute__((visibility("default")))
-
-namespace {
-
-constexpr int kWave = 64;
-constexpr int kWG = 256;                 // threads per workgroup (4 waves)
-constexpr int kWavesPerWG = kWG / kWave;
-constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
-constexpr int kDefaultShortMax = 1024;
-constexpr int kDefaultGroupLanes = 16;
-
-struct __align__(8) Pair {
-    float v;
-    int32_t i;
-};
-
-// ---------------------------------------------------------------------------
-// per-lane accumulation: lanes visit their elements in increasing index order,
-// so strict '<' keeps the leftmost minimum within a lane.  The sentinel is
-// (+inf, INT_MAX); an element equal to +inf never replaces it, which the
-// final fix-up (value +inf => index l) corrects: with no NaN, a +inf minimum
-// means every element of the range is +inf and the leftmost is l.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void upd(float x, int e, float &bv, int &bi) {
-    const bool t = x < bv;
-    bv = t ? x : bv;
-    bi = t ? e : bi;
-}
-
-__device__ __forceinline__ void upd4(const float4 &x, int e, float &bv, int &bi) {
-    upd(x.x, e, bv, bi);
-    upd(x.y, e + 1, bv, bi);
-    upd(x.z, e + 2, bv, bi);
-    upd(x.w, e + 3, bv, bi);
-}
-
-// (ov, oi) replaces (v, i) when smaller, or equal with a smaller index.
-__device__ __forceinline__ void take_if_better(float ov, int oi, float &v, int &i) {
-    const bool t = (ov < v) || (ov == v && oi < i);
-    v = t ? ov : v;
-    i = t ? oi : i;
-}
-
-// One DPP exchange step.  'old' is the lane's own value, so a lane whose
-// source is out of the row keeps its own pair (idempotent for a minimum).
-template <int CTRL>
-__device__ __forceinline__ void dpp_step(float &v, int &i) {
-    const int ov = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false);
-    const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false);
-    take_if_better(__int_as_float(ov), oi, v, i);
-}
-
-// DPP controls: quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2),
-// row_half_mirror (pairs the two quads of each 8), row_mirror (pairs the two
-// halves of each row of 16).  After G steps every lane of an aligned group of
-// G lanes holds the group's minimum.
-template <int G>
-__device__ __forceinline__ void group_reduce(float &v, int &i) {
-    static_assert(G == 4 || G == 8 || G == 16, "DPP group reduction works within a row of 16");
-    dpp_step<0xB1>(v, i);
-    dpp_step<0x4E>(v, i);
-    if constexpr (G >= 8) dpp_step<0x141>(v, i);
-    if constexpr (G >= 16) dpp_step<0x140>(v, i);
-}
-
-// Whole-wave reduction: rows of 16 by DPP, then the four row results are read
-// into scalar registers.  Every lane returns the same pair.
-__device__ __forceinline__ void wave_reduce(float &v, int &i) {
-    group_reduce<16>(v, i);
-    float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    int ri = __builtin_amdgcn_readlane(i, 0);
-#pragma unroll
-    for (int row = 1; row < 4; ++row) {
-        const float ov = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), row * 16));
-        const int oi = __builtin_amdgcn_readlane(i, row * 16);
-        take_if_better(ov, oi, rv, ri);
-    }
-    v = rv;
-    i = ri;
-}
-
-__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
-
-__device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Scans A[a..b] (a <= b) with STRIDE cooperating lanes, lane in [0, STRIDE).
-// Head (up to 3 elements before the first 16-byte boundary) and tail (up to
-// 3 after the last) are scalar loads; the aligned middle is float4 loads, U
-// per lane in flight.  Each lane visits head, middle, tail in index order.
-// Every load lies inside [a, b], so nothing past the array end is touched.
-template <int STRIDE, int U>
-__device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
-    const int ah = (a + 3) & ~3;
-    const int bt = (b + 1) & ~3;
-    const int h_end = min(ah, b + 1);
-    const int t_beg = max(bt, h_end);
-    if (lane < h_end - a) upd(A[a + lane], a + lane, bv, bi);
-    const float4 *__restrict__ A4 = reinterpret_cast<const float4 *>(A);
-    int v = (h_end >> 2) + lane;
-    const int v_end = t_beg >> 2;  // exclusive; empty when h_end == t_beg
-    if (h_end < t_beg) {
-        for (; v + (U - 1) * STRIDE < v_end; v += U * STRIDE) {
-            float4 x[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) x[u] = A4[v + u * STRIDE];
-#pragma unroll
-            for (int u = 0; u < U; ++u) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
-        }
-        for (; v < v_end; v += STRIDE) upd4(A4[v], v << 2, bv, bi);
-    }
-    if (lane < b + 1 - t_beg) upd(A[t_beg + lane], t_beg + lane, bv, bi);
-}
+namespace rmqx {
 
 // ---------------------------------------------------------------------------
 // build kernels
@@ -188,152 +75,24 @@ __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__
     cur[i] = (y.v < x.v) ? y : x;
 }
 
-// ---------------------------------------------------------------------------
-// query kernels
-//
-// Binning is done per tile of 64 consecutive queries, in registers: lane j of
-// a wave loads query t*64+j, and two ballots split the tile into short ranges
-// (at most short_max elements) and long ones.  The short kernel answers the
-// short ranges of every tile, several per wave; the long kernel answers the
-// long ranges of every tile, one whole wave each.  Both read the query array
-// once, so binning costs 8 bytes per query per kernel and no atomics, no
-// queue and no workspace: each query writes only its own output slot.
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
-    if (out_v) out_v[qi] = v;
-    if (out_i) out_i[qi] = i;
-}
-
-struct TileMasks {
-    int l, r;          // this lane's query
-    uint64_t shorts;   // lanes holding a valid range of at most short_max elements
-    uint64_t longs;    // lanes holding a valid longer range
-};
-
-__device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64_t q, int n, int short_max, int64_t t,
-                                               int lane, bool flag_invalid, float *out_v, int32_t *out_i) {
-    const int64_t qi = t * kWave + lane;
-    const bool in = qi < q;
-    int l = 0, r = -1;
-    if (in) {
-        const int2 p = Q[qi];
-        l = p.x;
-        r = p.y;
-    }
-    const bool ok = in && l >= 0 && l <= r && r < n;
-    if (flag_invalid && in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
-    const bool is_short = ok && (r - l) < short_max;
-    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
-}
-
-// One query [l, r] answered by the G lanes of a group (gl = lane in group).
-// Short class: one direct scan.  Long class: lB == rB is one in-block scan
-// (src/rtx_kernels.cu:116-135); otherwise the left edge block from l and the
-// right edge block up to r (:151-177) are scanned and, when whole blocks lie
-// between, the sparse table over block minima (:138-149) gives their leftmost
-// minimum in O(1).  The group's result is combined with the table's with the
-// index as tie-break, so the leftmost minimum wins.
-template <int G, int U, bool LONG>
-__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, int nb,
-                                               const Pair *__restrict__ st, int l, int r, int gl, float &bv,
-                                               int &bi, float &mv, int &mi) {
-    if constexpr (!LONG) {
-        scan_range<G, U>(A, l, r, gl, bv, bi);
-    } else {
-        const int lB = l / bs, rB = r / bs;
-        if (lB == rB) {
-            scan_range<G, U>(A, l, r, gl, bv, bi);
-        } else {
-            if (rB - lB > 1) {
-                const int a = lB + 1, c = rB - lB - 1;
-                const int lev = 31 - __clz(c);
-                const Pair *row = st + (int64_t)lev * nb;
-                const Pair x = row[a];
-                const Pair y = row[a + c - (1 << lev)];
-                const Pair m = (y.v < x.v) ? y : x;
-                mv = m.v;
-                mi = m.i;
-            }
-            scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);
-            scan_range<G, U>(A, rB * bs, r, gl, bv, bi);
-        }
-    }
-}
-
 // Tile kernel: each wave takes tiles of 64 consecutive queries and answers
-// the ones of its class (LONG = more than short_max elements), G lanes per
-// query.  G == 64: the whole wave per query, queries taken from the ballot
-// mask in lane order.  G < 64: the class's queries are staged in LDS and
-// answered 64/G per round, every lane taking part in each round's DPP
-// reduction.  The short-class kernel also writes (NaN, -1) for invalid
+// the ones of its class (LONG = more than short_max elements) with
+// answer_class.  The short-class kernel also writes (NaN, -1) for invalid
 // queries.  Persistent grid, static stride over tiles.
 template <int G, bool LONG>
 __global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A, int n, int bs, int nb,
                                                      const Pair *__restrict__ st, const int2 *__restrict__ Q,
                                                      int64_t q, int short_max, float *__restrict__ out_v,
                                                      int32_t *__restrict__ out_i) {
-    constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
+    __shared__ TileStage stage[kWavesPerWG];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
-    if constexpr (G == kWave) {
-        for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-            const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
-            for (uint64_t mk = LONG ? m.longs : m.shorts; mk; mk &= mk - 1) {
-                const int src = __ffsll((unsigned long long)mk) - 1;
-                const int l = __builtin_amdgcn_readlane(m.l, src);
-                const int r = __builtin_amdgcn_readlane(m.r, src);
-                float bv = INFINITY, mv = INFINITY;
-                int bi = INT_MAX, mi = INT_MAX;
-                answer_partial<kWave, U, LONG>(A, bs, nb, st, l, r, lane, bv, bi, mv, mi);
-                wave_reduce(bv, bi);
-                take_if_better(mv, mi, bv, bi);
-                if (lane == 0) {
-                    if (bv == INFINITY) bi = l;
-                    write_result(out_v, out_i, t * kWave + src, bv, bi);
-                }
-            }
-        }
-    } else {
-        constexpr int QPR = kWave / G;  // queries per round
-        __shared__ int s_l[kWavesPerWG][kWave];
-        __shared__ int s_r[kWavesPerWG][kWave];
-        __shared__ int s_q[kWavesPerWG][kWave];
-        const int g = lane / G;
-        const int gl = lane & (G - 1);
-        for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-            const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
-            const uint64_t mk = LONG ? m.longs : m.shorts;
-            if ((mk >> lane) & 1) {
-                const uint32_t k = mask_rank(mk);
-                s_l[wv][k] = m.l;
-                s_r[wv][k] = m.r;
-                s_q[wv][k] = lane;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            const int count = __popcll(mk);
-            for (int base = 0; base < count; base += QPR) {
-                const int s = base + g;
-                const bool has = s < count;
-                float bv = INFINITY, mv = INFINITY;
-                int bi = INT_MAX, mi = INT_MAX;
-                int ql = 0;
-                if (has) {
-                    ql = s_l[wv][s];
-                    answer_partial<G, U, LONG>(A, bs, nb, st, ql, s_r[wv][s], gl, bv, bi, mv, mi);
-                }
-                group_reduce<G>(bv, bi);
-                take_if_better(mv, mi, bv, bi);
-                if (has && gl == 0) {
-                    if (bv == INFINITY) bi = ql;
-                    write_result(out_v, out_i, t * kWave + s_q[wv][s], bv, bi);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
+    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
+        answer_class<G, LONG>(A, bs, nb, st, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
+                              stage[wv]);
     }
 }
 
@@ -412,81 +171,13 @@ __global__ void k_gen_array(float *__restrict__ A, int64_t n, uint64_t key, int 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) A[i] = gen_value(key, i, kind);
 }
 
+}  // namespace rmqx
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
-int g_num_cus = 0;
-
-int device_cus() {
-    if (g_num_cus > 0) return g_num_cus;
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 256;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    g_num_cus = cus;
-    return cus;
-}
-
-int hip_rc(hipError_t e) {
-    if (e == hipSuccess) return RMQ_OK;
-    if (e == hipErrorOutOfMemory) return RMQ_ERR_OOM;
-    (void)hipGetLastError();
-    return RMQ_ERR_HIP;
-}
-
-#define RMQ_HIP(call)                         \
-    do {                                      \
-        const int rc_ = hip_rc((call));       \
-        if (rc_ != RMQ_OK) return rc_;        \
-    } while (0)
-
-}  // namespace
-
-struct rmq_handle {
-    const float *A;
-    int64_t n;
-    int32_t bs;
-    int64_t nb;
-    int32_t levels;
-    int device;
-    Pair *st;  // levels * nb pairs; level 0 = block minima
-    int64_t st_bytes;
-};
-
-namespace {
-
-int check_device(const rmq_handle *h) {
-    int dev = -1;
-    if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
-    return dev == h->device ? RMQ_OK : RMQ_ERR_DEVICE;
-}
-
-void fill_default_opts(const rmq_handle *h, rmq_query_opts_t *o) {
-    (void)h;
-    o->short_max = kDefaultShortMax;
-    o->group_lanes = kDefaultGroupLanes;
-    o->long_wgs = 0;
-    o->long_lanes = kWave;
-}
-
-int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts_t *o) {
-    fill_default_opts(h, o);
-    if (in) {
-        if (in->short_max < 0 || in->long_wgs < 0) return RMQ_ERR_INVALID_ARG;
-        if (in->short_max > 0) o->short_max = in->short_max;
-        auto lanes_ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 64; };
-        if (in->group_lanes != 0) {
-            if (!lanes_ok(in->group_lanes)) return RMQ_ERR_INVALID_ARG;
-            o->group_lanes = in->group_lanes;
-        }
-        if (in->long_lanes != 0) {
-            if (!lanes_ok(in->long_lanes)) return RMQ_ERR_INVALID_ARG;
-            o->long_lanes = in->long_lanes;
-        }
-        if (in->long_wgs > 0) o->long_wgs = in->long_wgs;
-    }
-    return RMQ_OK;
-}
+namespace rmqx {
 
 int tiles_grid(int64_t q, int per_cu) {
     const int64_t tiles = (q + kWave - 1) / kWave;
@@ -558,7 +249,9 @@ int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val,
     return rc;
 }
 
-}  // namespace
+}  // namespace rmqx
+
+using namespace rmqx;
 
 // ===========================================================================
 // C ABI
@@ -689,66 +382,96 @@ RMQ_API int rmq_gen_array_host(float *A, int64_t n, uint64_t seed, int32_t kind)
     return RMQ_OK;
 }
 
-namespace {
+namespace rmqx {
 
-// Box-Muller in double; contraction off so every host build rounds alike.
-double lognormal_len(uint64_t k3, uint64_t k4, uint64_t i, double mu, double sigma) {
+// One query stream: counter-based, the same definition on host and device.
+struct QueryGen {
+    uint64_t k1, k2, k3, k4;
+    double mu, sigma;
+    int32_t n, lr;
+};
+
+// lr codes (after src/rand.cuh:179-201): > 0 fixed length; -1 uniform length;
+// -2 / -3 lognormal medians 2^14 / 2^6, sigma 1; -4 / -5 the reference's
+// scales n/2^8, n/2^15 with sigma 0.3; -6 r = n-1; -7 / -8 lognormal with the
+// -2 / -3 medians scaled to n (median n^(14/log2 1e8), n^(6/log2 1e8)).
+inline bool make_query_gen(int32_t n, int32_t lr, uint64_t seed, QueryGen *g) {
+    if (n <= 0 || lr == 0 || lr > n || lr < -8) return false;
+    g->k1 = rng_key(seed, 1);
+    g->k2 = rng_key(seed, 2);
+    g->k3 = rng_key(seed, 3);
+    g->k4 = rng_key(seed, 4);
+    g->n = n;
+    g->lr = lr;
+    g->mu = 0.0;
+    g->sigma = 1.0;
+    if (lr == -2) g->mu = 14.0 * 0.69314718055994530942;
+    if (lr == -3) g->mu = 6.0 * 0.69314718055994530942;
+    if (lr == -4 || lr == -5) {
+        const int64_t sc = n / (lr == -4 ? 256 : 32768);
+        g->mu = std::log((double)(sc > 1 ? sc : 1));
+        g->sigma = 0.3;
+    }
+    if (lr == -7) g->mu = 14.0 / std::log2(1e8) * std::log((double)n);
+    if (lr == -8) g->mu = 6.0 / std::log2(1e8) * std::log((double)n);
+    return true;
+}
+
+__host__ __device__ inline int64_t below(uint64_t r, uint64_t m) {
+    return (int64_t)(((unsigned __int128)r * m) >> 64);
+}
+
+// Box-Muller in double with contraction off, so every build rounds alike
+// (host builds bit for bit; the device's libm may differ in the last ulp).
+__host__ __device__ inline double lognormal_len(uint64_t k3, uint64_t k4, uint64_t i, double mu, double sigma) {
 #pragma clang fp contract(off)
     const double u1 = (double)((rng_at(k3, i) >> 11) + 1) * 0x1p-53;  // (0,1]
     const double u2 = (double)(rng_at(k4, i) >> 11) * 0x1p-53;        // [0,1)
-    const double rad = std::sqrt(-2.0 * std::log(u1));
+    const double rad = sqrt(-2.0 * log(u1));
     const double ang = 6.283185307179586476925286766559 * u2;
-    const double z = rad * std::cos(ang);
+    const double z = rad * cos(ang);
     const double sz = sigma * z;
     const double e = mu + sz;
-    return std::exp(e);
+    return exp(e);
 }
 
-inline int64_t below(uint64_t r, uint64_t m) { return (int64_t)(((unsigned __int128)r * m) >> 64); }
+__host__ __device__ inline void gen_query(const QueryGen &g, uint64_t i, int32_t *out) {
+    int64_t l, len;
+    if (g.lr > 0) {
+        len = g.lr;
+        l = below(rng_at(g.k1, i), (uint64_t)(g.n - g.lr + 1));
+    } else {
+        l = below(rng_at(g.k1, i), (uint64_t)g.n);
+        const int64_t room = (int64_t)g.n - l;
+        if (g.lr == -1) {
+            len = 1 + below(rng_at(g.k2, i), (uint64_t)room);
+        } else if (g.lr == -6) {
+            len = room;
+        } else {
+            double x = lognormal_len(g.k3, g.k4, i, g.mu, g.sigma);
+            if (!(x < 4.0e18)) x = 4.0e18;
+            len = llround(x);
+            if (len < 1) len = 1;
+            if (len > room) len = room;
+        }
+    }
+    out[0] = (int32_t)l;
+    out[1] = (int32_t)(l + len - 1);
+}
 
-}  // namespace
+__global__ void k_gen_queries(int32_t *__restrict__ Q, int64_t q, QueryGen g) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q; i += stride)
+        gen_query(g, (uint64_t)i, Q + 2 * i);
+}
+
+}  // namespace rmqx
 
 RMQ_API int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int32_t nthreads) {
-    if (!Q || q < 0 || n <= 0 || lr == 0 || lr > n || lr < -6) return RMQ_ERR_INVALID_ARG;
-    const uint64_t k1 = rng_key(seed, 1), k2 = rng_key(seed, 2), k3 = rng_key(seed, 3), k4 = rng_key(seed, 4);
-    double mu = 0.0, sigma = 1.0;
-    if (lr == -2) mu = 14.0 * 0.69314718055994530942;
-    if (lr == -3) mu = 6.0 * 0.69314718055994530942;
-    if (lr == -4) {
-        const int64_t sc = n / 256;
-        mu = std::log((double)(sc > 1 ? sc : 1));
-        sigma = 0.3;
-    }
-    if (lr == -5) {
-        const int64_t sc = n / 32768;
-        mu = std::log((double)(sc > 1 ? sc : 1));
-        sigma = 0.3;
-    }
+    QueryGen g;
+    if (!Q || q < 0 || !make_query_gen(n, lr, seed, &g)) return RMQ_ERR_INVALID_ARG;
     auto work = [&](int64_t lo, int64_t hi) {
-        for (int64_t i = lo; i < hi; ++i) {
-            const uint64_t ui = (uint64_t)i;
-            int64_t l, len;
-            if (lr > 0) {
-                len = lr;
-                l = below(rng_at(k1, ui), (uint64_t)(n - lr + 1));
-            } else {
-                l = below(rng_at(k1, ui), (uint64_t)n);
-                const int64_t room = (int64_t)n - l;
-                if (lr == -1) {
-                    len = 1 + below(rng_at(k2, ui), (uint64_t)room);
-                } else if (lr == -6) {
-                    len = room;
-                } else {
-                    double x = lognormal_len(k3, k4, ui, mu, sigma);
-                    if (!(x < 4.0e18)) x = 4.0e18;
-                    len = std::llround(x);
-                    if (len < 1) len = 1;
-                    if (len > room) len = room;
-                }
-            }
-            Q[2 * i] = (int32_t)l;
-            Q[2 * i + 1] = (int32_t)(l + len - 1);
-        }
+        for (int64_t i = lo; i < hi; ++i) gen_query(g, (uint64_t)i, Q + 2 * i);
     };
     int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
     if (q < 65536) nt = 1;
@@ -765,6 +488,15 @@ RMQ_API int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, u
     }
     for (auto &th : pool) th.join();
     return RMQ_OK;
+}
+
+RMQ_API int rmq_gen_queries(int32_t *d_Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, hipStream_t s) {
+    QueryGen g;
+    if (q < 0 || (q > 0 && !d_Q) || !make_query_gen(n, lr, seed, &g)) return RMQ_ERR_INVALID_ARG;
+    if (q == 0) return RMQ_OK;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((q + 255) / 256, (int64_t)device_cus() * 32));
+    k_gen_queries<<<grid, 256, 0, s>>>(d_Q, q, g);
+    return hip_rc(hipGetLastError());
 }
 
 // ===========================================================================
@@ -856,7 +588,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     if (n_bs > 0) {
         cands.assign(bs_list, bs_list + n_bs);
     } else {
-        for (int b = 5; b <= 16; ++b) cands.push_back(1 << b);
+        for (int b = 3; b <= 16; ++b) cands.push_back(1 << b);
     }
     auto base_opts = [](int32_t bs) {
         rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, bs <= 2048 ? 16 : kWave};

@@ -1,0 +1,347 @@
+// rmq_common.h -- device helpers and host plumbing shared by the engine's
+// HIP sources (rmq.hip: single-array build and query; batch.hip: work-item
+// plans, the persistent multi-array kernel and the serving loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+
+#include "rmq.h"
+
+#define RMQ_API extern "C" __attribute__((visibility("default")))
+
+namespace rmqx {
+
+constexpr int kWave = 64;
+constexpr int kWG = 256;                 // threads per workgroup (4 waves)
+constexpr int kWavesPerWG = kWG / kWave;
+constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
+constexpr int kDefaultShortMax = 1024;
+constexpr int kDefaultGroupLanes = 16;
+
+struct __align__(8) Pair {
+    float v;
+    int32_t i;
+};
+
+// ---------------------------------------------------------------------------
+// per-lane accumulation: lanes visit their elements in increasing index order,
+// so strict '<' keeps the leftmost minimum within a lane.  The sentinel is
+// (+inf, INT_MAX); an element equal to +inf never replaces it, which the
+// final fix-up (value +inf => index l) corrects: with no NaN, a +inf minimum
+// means every element of the range is +inf and the leftmost is l.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void upd(float x, int e, float &bv, int &bi) {
+    const bool t = x < bv;
+    bv = t ? x : bv;
+    bi = t ? e : bi;
+}
+
+__device__ __forceinline__ void upd4(const float4 &x, int e, float &bv, int &bi) {
+    upd(x.x, e, bv, bi);
+    upd(x.y, e + 1, bv, bi);
+    upd(x.z, e + 2, bv, bi);
+    upd(x.w, e + 3, bv, bi);
+}
+
+// (ov, oi) replaces (v, i) when smaller, or equal with a smaller index.
+__device__ __forceinline__ void take_if_better(float ov, int oi, float &v, int &i) {
+    const bool t = (ov < v) || (ov == v && oi < i);
+    v = t ? ov : v;
+    i = t ? oi : i;
+}
+
+// One DPP exchange step.  'old' is the lane's own value, so a lane whose
+// source is out of the row keeps its own pair (idempotent for a minimum).
+template <int CTRL>
+__device__ __forceinline__ void dpp_step(float &v, int &i) {
+    const int ov = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false);
+    take_if_better(__int_as_float(ov), oi, v, i);
+}
+
+// DPP controls: quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2),
+// row_half_mirror (pairs the two quads of each 8), row_mirror (pairs the two
+// halves of each row of 16).  After G steps every lane of an aligned group of
+// G lanes holds the group's minimum.
+template <int G>
+__device__ __forceinline__ void group_reduce(float &v, int &i) {
+    static_assert(G == 4 || G == 8 || G == 16, "DPP group reduction works within a row of 16");
+    dpp_step<0xB1>(v, i);
+    dpp_step<0x4E>(v, i);
+    if constexpr (G >= 8) dpp_step<0x141>(v, i);
+    if constexpr (G >= 16) dpp_step<0x140>(v, i);
+}
+
+// Whole-wave reduction: rows of 16 by DPP, then the four row results are read
+// into scalar registers.  Every lane returns the same pair.
+__device__ __forceinline__ void wave_reduce(float &v, int &i) {
+    group_reduce<16>(v, i);
+    float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const float ov = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), row * 16));
+        const int oi = __builtin_amdgcn_readlane(i, row * 16);
+        take_if_better(ov, oi, rv, ri);
+    }
+    v = rv;
+    i = ri;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Scans A[a..b] (a <= b) with STRIDE cooperating lanes, lane in [0, STRIDE).
+// Head (up to 3 elements before the first 16-byte boundary) and tail (up to
+// 3 after the last) are scalar loads; the aligned middle is float4 loads, U
+// per lane in flight.  Each lane visits head, middle, tail in index order.
+// Every load lies inside [a, b], so nothing past the array end is touched.
+template <int STRIDE, int U>
+__device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
+    const int ah = (a + 3) & ~3;
+    const int bt = (b + 1) & ~3;
+    const int h_end = min(ah, b + 1);
+    const int t_beg = max(bt, h_end);
+    if (lane < h_end - a) upd(A[a + lane], a + lane, bv, bi);
+    const float4 *__restrict__ A4 = reinterpret_cast<const float4 *>(A);
+    int v = (h_end >> 2) + lane;
+    const int v_end = t_beg >> 2;  // exclusive; empty when h_end == t_beg
+    if (h_end < t_beg) {
+        for (; v + (U - 1) * STRIDE < v_end; v += U * STRIDE) {
+            float4 x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = A4[v + u * STRIDE];
+#pragma unroll
+            for (int u = 0; u < U; ++u) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
+        }
+        for (; v < v_end; v += STRIDE) upd4(A4[v], v << 2, bv, bi);
+    }
+    if (lane < b + 1 - t_beg) upd(A[t_beg + lane], t_beg + lane, bv, bi);
+}
+
+// ---------------------------------------------------------------------------
+// query kernels
+//
+// Binning is done per tile of 64 consecutive queries, in registers: lane j of
+// a wave loads query t*64+j, and two ballots split the tile into short ranges
+// (at most short_max elements) and long ones.  The short kernel answers the
+// short ranges of every tile, several per wave; the long kernel answers the
+// long ranges of every tile, one whole wave each.  Both read the query array
+// once, so binning costs 8 bytes per query per kernel and no atomics, no
+// queue and no workspace: each query writes only its own output slot.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
+    if (out_v) out_v[qi] = v;
+    if (out_i) out_i[qi] = i;
+}
+
+struct TileMasks {
+    int l, r;          // this lane's query
+    uint64_t shorts;   // lanes holding a valid range of at most short_max elements
+    uint64_t longs;    // lanes holding a valid longer range
+};
+
+__device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64_t q, int n, int short_max, int64_t t,
+                                               int lane, bool flag_invalid, float *out_v, int32_t *out_i) {
+    const int64_t qi = t * kWave + lane;
+    const bool in = qi < q;
+    int l = 0, r = -1;
+    if (in) {
+        const int2 p = Q[qi];
+        l = p.x;
+        r = p.y;
+    }
+    const bool ok = in && l >= 0 && l <= r && r < n;
+    if (flag_invalid && in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    const bool is_short = ok && (r - l) < short_max;
+    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+}
+
+// One query [l, r] answered by the G lanes of a group (gl = lane in group).
+// Short class: one direct scan.  Long class: lB == rB is one in-block scan
+// (src/rtx_kernels.cu:116-135); otherwise the left edge block from l and the
+// right edge block up to r (:151-177) are scanned and, when whole blocks lie
+// between, the sparse table over block minima (:138-149) gives their leftmost
+// minimum in O(1).  The group's result is combined with the table's with the
+// index as tie-break, so the leftmost minimum wins.
+template <int G, int U, bool LONG>
+__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, int nb,
+                                               const Pair *__restrict__ st, int l, int r, int gl, float &bv,
+                                               int &bi, float &mv, int &mi) {
+    if constexpr (!LONG) {
+        scan_range<G, U>(A, l, r, gl, bv, bi);
+    } else {
+        const int lB = l / bs, rB = r / bs;
+        if (lB == rB) {
+            scan_range<G, U>(A, l, r, gl, bv, bi);
+        } else {
+            if (rB - lB > 1) {
+                const int a = lB + 1, c = rB - lB - 1;
+                const int lev = 31 - __clz(c);
+                const Pair *row = st + (int64_t)lev * nb;
+                const Pair x = row[a];
+                const Pair y = row[a + c - (1 << lev)];
+                const Pair m = (y.v < x.v) ? y : x;
+                mv = m.v;
+                mi = m.i;
+            }
+            scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);
+            scan_range<G, U>(A, rB * bs, r, gl, bv, bi);
+        }
+    }
+}
+
+// Per-wave LDS staging of one tile's queries of a class.
+struct TileStage {
+    int l[kWave];
+    int r[kWave];
+    int lane[kWave];
+};
+
+// Answers the queries of one class in a tile (mask mk over the tile's lanes),
+// G lanes per query; results go to out[qbase + lane].  G == 64: the whole
+// wave per query, queries taken from the mask in lane order.  G < 64: the
+// queries are staged in LDS and answered 64/G per round, every lane taking
+// part in each round's DPP reduction.  Must be called by the whole wave.
+template <int G, bool LONG>
+__device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs, int nb, const Pair *__restrict__ st,
+                                             uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
+                                             float *__restrict__ out_v, int32_t *__restrict__ out_i,
+                                             TileStage &stage) {
+    constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
+    if constexpr (G == kWave) {
+        for (; mk; mk &= mk - 1) {
+            const int src = __ffsll((unsigned long long)mk) - 1;
+            const int l = __builtin_amdgcn_readlane(m.l, src);
+            const int r = __builtin_amdgcn_readlane(m.r, src);
+            float bv = INFINITY, mv = INFINITY;
+            int bi = INT_MAX, mi = INT_MAX;
+            answer_partial<kWave, U, LONG>(A, bs, nb, st, l, r, lane, bv, bi, mv, mi);
+            wave_reduce(bv, bi);
+            take_if_better(mv, mi, bv, bi);
+            if (lane == 0) {
+                if (bv == INFINITY) bi = l;
+                write_result(out_v, out_i, qbase + src, bv, bi);
+            }
+        }
+    } else {
+        constexpr int QPR = kWave / G;  // queries per round
+        const int g = lane / G;
+        const int gl = lane & (G - 1);
+        if ((mk >> lane) & 1) {
+            const uint32_t k = mask_rank(mk);
+            stage.l[k] = m.l;
+            stage.r[k] = m.r;
+            stage.lane[k] = lane;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int count = __popcll(mk);
+        for (int base = 0; base < count; base += QPR) {
+            const int s = base + g;
+            const bool has = s < count;
+            float bv = INFINITY, mv = INFINITY;
+            int bi = INT_MAX, mi = INT_MAX;
+            int ql = 0;
+            if (has) {
+                ql = stage.l[s];
+                answer_partial<G, U, LONG>(A, bs, nb, st, ql, stage.r[s], gl, bv, bi, mv, mi);
+            }
+            group_reduce<G>(bv, bi);
+            take_if_better(mv, mi, bv, bi);
+            if (has && gl == 0) {
+                if (bv == INFINITY) bi = ql;
+                write_result(out_v, out_i, qbase + stage.lane[s], bv, bi);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host plumbing
+// ---------------------------------------------------------------------------
+inline int device_cus() {
+    static int g_num_cus = 0;
+    if (g_num_cus > 0) return g_num_cus;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    g_num_cus = cus;
+    return cus;
+}
+
+inline int hip_rc(hipError_t e) {
+    if (e == hipSuccess) return RMQ_OK;
+    if (e == hipErrorOutOfMemory) return RMQ_ERR_OOM;
+    (void)hipGetLastError();
+    return RMQ_ERR_HIP;
+}
+
+#define RMQ_HIP(call)                         \
+    do {                                      \
+        const int rc_ = hip_rc((call));       \
+        if (rc_ != RMQ_OK) return rc_;        \
+    } while (0)
+
+}  // namespace rmqx
+
+struct rmq_handle {
+    const float *A;
+    int64_t n;
+    int32_t bs;
+    int64_t nb;
+    int32_t levels;
+    int device;
+    rmqx::Pair *st;  // levels * nb pairs; level 0 = block minima
+    int64_t st_bytes;
+};
+
+namespace rmqx {
+
+inline int check_device(const rmq_handle *h) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
+    return dev == h->device ? RMQ_OK : RMQ_ERR_DEVICE;
+}
+
+inline void fill_default_opts(const rmq_handle *h, rmq_query_opts_t *o) {
+    (void)h;
+    o->short_max = kDefaultShortMax;
+    o->group_lanes = kDefaultGroupLanes;
+    o->long_wgs = 0;
+    o->long_lanes = kWave;
+}
+
+inline int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts_t *o) {
+    fill_default_opts(h, o);
+    if (in) {
+        if (in->short_max < 0 || in->long_wgs < 0) return RMQ_ERR_INVALID_ARG;
+        if (in->short_max > 0) o->short_max = in->short_max;
+        auto lanes_ok = [](int g) { return g == 4 || g == 8 || g == 16 || g == 64; };
+        if (in->group_lanes != 0) {
+            if (!lanes_ok(in->group_lanes)) return RMQ_ERR_INVALID_ARG;
+            o->group_lanes = in->group_lanes;
+        }
+        if (in->long_lanes != 0) {
+            if (!lanes_ok(in->long_lanes)) return RMQ_ERR_INVALID_ARG;
+            o->long_lanes = in->long_lanes;
+        }
+        if (in->long_wgs > 0) o->long_wgs = in->long_wgs;
+    }
+    return RMQ_OK;
+}
+
+int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
+               const rmq_query_opts_t *opts_in, hipStream_t s, rmq_query_times_t *times);
+
+}  // namespace rmqx

@@ -198,6 +198,69 @@ def gen_array(n: int, seed: int, kind: str = "uniform", device="cuda", stream=No
     return A
 
 
+def gen_queries_device(q: int, n: int, lr: int, seed: int, device="cuda", stream=None):
+    """Device queries from the same counter-based stream (rmq_gen_queries); (q, 2) int32 tensor."""
+    torch = _torch()
+    Q = torch.empty((q, 2), dtype=torch.int32, device=device)
+    N.check(N.lib().rmq_gen_queries(Q.data_ptr(), q, n, lr, seed, _stream_ptr(stream)), "rmq_gen_queries")
+    return Q
+
+
+class Plan:
+    """Many arrays (or requests) answered by one persistent launch (rmq_plan_*).
+
+    indexes[i] answers Qs[i] into out_idx[i] and/or out_val[i].  The tensors
+    are referenced for the plan's lifetime (the device work list holds their
+    addresses); their contents may change between launches.
+    """
+
+    def __init__(self, indexes, Qs, out_idx=None, out_val=None, opts: N.QueryOpts | None = None,
+                 item_queries: int = 0, stream=None):
+        m = len(indexes)
+        if len(Qs) != m or (out_idx is not None and len(out_idx) != m) or (out_val is not None and len(out_val) != m):
+            raise ValueError("indexes, Qs and outputs must have the same length")
+        self._keep = (list(indexes), [_dev_queries(Q) for Q in Qs], out_idx, out_val)
+        P = ctypes.c_void_p * m
+        hs = P(*[ix.handle for ix in indexes])
+        qp = P(*[Q.data_ptr() for Q in Qs])
+        qs = (ctypes.c_int64 * m)(*[Q.numel() // 2 for Q in Qs])
+        ip = P(*[t.data_ptr() if t is not None else None for t in out_idx]) if out_idx is not None else None
+        vp = P(*[t.data_ptr() if t is not None else None for t in out_val]) if out_val is not None else None
+        h = ctypes.c_void_p()
+        rc = N.lib().rmq_plan_create(hs, m, qp, qs, vp, ip, ctypes.byref(opts) if opts is not None else None,
+                                     item_queries, _stream_ptr(stream), ctypes.byref(h))
+        N.check(rc, "rmq_plan_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        N.check(N.lib().rmq_plan_launch(self._h, _stream_ptr(stream)), "rmq_plan_launch")
+
+    def graph_launch(self, stream=None):
+        N.check(N.lib().rmq_plan_graph_launch(self._h, _stream_ptr(stream)), "rmq_plan_graph_launch")
+
+    def info(self) -> dict:
+        a, it, q = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        N.check(N.lib().rmq_plan_info(self._h, ctypes.byref(a), ctypes.byref(it), ctypes.byref(q)), "rmq_plan_info")
+        return {"arrays": a.value, "items": it.value, "queries": q.value}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().rmq_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def gen_array_host(n: int, seed: int, kind: str = "uniform") -> np.ndarray:
     A = np.empty(n, np.float32)
     N.check(N.lib().rmq_gen_array_host(A.ctypes.data, n, seed, N.GEN_KINDS[kind]), "rmq_gen_array_host")

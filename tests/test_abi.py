@@ -47,17 +47,25 @@ def test_argument_validation_before_device_use():
     assert lib.rmq_gen_array(fake, 10, 1, 9, None) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_query_check(fake, 1, 10, None, None) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_handle_info(None, None, None, None, None, None) == R.RMQ_ERR_INVALID_ARG
+    plan = ctypes.c_void_p()
+    assert lib.rmq_plan_create(None, -1, None, None, None, None, None, 0, None, ctypes.byref(plan)) == \
+        R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_plan_create(None, 2, None, None, None, None, None, 0, None, ctypes.byref(plan)) == \
+        R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_plan_launch(None, None) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_gen_queries(None, 5, 10, -1, 1, None) == R.RMQ_ERR_INVALID_ARG
     lib.rmq_free(None)  # no-op
+    lib.rmq_plan_free(None)  # no-op
 
 
 def test_query_generator_rejects_bad_specs():
     Q = np.empty((4, 2), np.int32)
     lib = R.lib()
-    for n, lr in [(0, -1), (10, 0), (10, 11), (10, -7)]:
+    for n, lr in [(0, -1), (10, 0), (10, 11), (10, -9)]:
         assert lib.rmq_gen_queries_host(Q.ctypes.data, 4, n, lr, 1, 1) == R.RMQ_ERR_INVALID_ARG
 
 
-@pytest.mark.parametrize("lr", [-1, -2, -3, -4, -5, -6, 1, 1000])
+@pytest.mark.parametrize("lr", [-1, -2, -3, -4, -5, -6, -7, -8, 1, 1000])
 def test_product_query_generator_matches_oracle(lr):
     a = R.gen_queries(200000, 10 ** 8, lr, 1321, nthreads=4)
     b = O.gen_queries(200000, 10 ** 8, lr, 1321, nt=3)

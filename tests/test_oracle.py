@@ -128,7 +128,7 @@ class TestGenerators:
         assert np.all(np.diff(d) < 0) and d.min() > 0
         assert np.all(O.gen_array(10, 2, "const") == 1.0)
 
-    @pytest.mark.parametrize("lr", [-1, -2, -3, -4, -5, -6, 1, 256])
+    @pytest.mark.parametrize("lr", [-1, -2, -3, -4, -5, -6, -7, -8, 1, 256])
     def test_queries_valid_and_deterministic(self, lr):
         n = 10 ** 6
         Q = O.gen_queries(100000, n, lr, 77, nt=4)
