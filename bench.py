@@ -273,14 +273,30 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=1 << 26)
     ap.add_argument("--bs", type=int, default=1 << 15, help="block size of the fixed-bs run")
     ap.add_argument("--fixed-steps", type=int, default=2, help="timed steps of the fixed-bs run")
-    ap.add_argument("--tune-sample", type=int, default=1 << 20, help="queries per batch the autotuner times")
+    ap.add_argument("--tune-sample", type=int, default=1 << 22, help="queries per batch the autotuner times")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "rtxrmq_amd", "tune_cache_mi355x.txt"),
+                    help="autotuner cache (picks per n and length signature); '' for none")
+    ap.add_argument("--retune", action="store_true", help="time the sweep even when the cache has a pick")
     ap.add_argument("--seed", type=int, default=1321)
     ap.add_argument("--cpu-seconds", type=float, default=1.5, help="wall seconds of HRMQ per batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        nel = args.copy_probe // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 copies of {nel * 4} bytes")
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries shard with no exchange "
@@ -320,11 +336,12 @@ def main() -> int:
     for lr in DISTS:
         sample = Q_dev[lr][: min(q, args.tune_sample)]
         t0 = time.perf_counter()
-        picks[lr] = R.tune(A, sample)
+        picks[lr] = R.tune(A, sample, cache_path=args.tune_cache or None, refresh=args.retune)
         p = picks[lr]
         log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} short_max={p['opts'].short_max} "
             f"group_lanes={p['opts'].group_lanes} long_lanes={p['opts'].long_lanes} "
-            f"({p['n_tried']} configs, {time.perf_counter() - t0:.1f} s, key {p['key']})")
+            f"({'cached' if p['from_cache'] else str(p['n_tried']) + ' configs'}, {time.perf_counter() - t0:.1f} s, "
+            f"key {p['key']})")
     indexes, build_ms = {}, {}
     for lr in DISTS:
         bs = picks[lr]["bs"]
@@ -371,7 +388,8 @@ def main() -> int:
                    "n": n, "q": q, "dists": [DIST_NAMES[lr] for lr in DISTS],
                    "bs": {DIST_NAMES[lr]: bs_for[lr] for lr in DISTS},
                    "construction_ms": {str(k): v for k, v in build_ms.items()},
-                   "tune_sample": min(q, args.tune_sample)},
+                   "tune_sample": min(q, args.tune_sample),
+                   "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},
         "roofline": {"bound": "hbm", "achieved": tuned["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tuned["achieved"] / HBM_PEAK_GBS,
                      "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"]},

@@ -26,6 +26,7 @@ namespace rmqx {
 struct ArrayDesc {
     const float *A;
     const Pair *st;
+    uint64_t magic;  // block_magic(bs)
     int32_t n, bs, nb, pad;
 };
 
@@ -60,10 +61,10 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
         const int64_t tiles = ((int64_t)w.count + kWave - 1) / kWave;
         for (int64_t t = 0; t < tiles; ++t) {
             const TileMasks m = load_tile(w.Q, w.count, d.n, short_max, t, lane, true, w.out_v, w.out_i);
-            answer_class<GS, false>(d.A, d.bs, d.nb, d.st, m.shorts, m, lane, t * kWave, w.out_v, w.out_i,
-                                    stage[wv]);
-            answer_class<GL, true>(d.A, d.bs, d.nb, d.st, m.longs, m, lane, t * kWave, w.out_v, w.out_i,
-                                   stage[wv]);
+            answer_class<GS, false>(d.A, d.bs, d.magic, d.nb, d.st, m.shorts, m, lane, t * kWave, w.out_v,
+                                    w.out_i, stage[wv]);
+            answer_class<GL, true>(d.A, d.bs, d.magic, d.nb, d.st, m.longs, m, lane, t * kWave, w.out_v,
+                                   w.out_i, stage[wv]);
         }
     }
 }
@@ -168,7 +169,7 @@ RMQ_API int rmq_plan_create(const rmq_handle_t *const *handles, int32_t m, const
             plan_release(p);
             return h && h->device != p->device ? RMQ_ERR_DEVICE : RMQ_ERR_INVALID_ARG;
         }
-        descs[i] = ArrayDesc{h->A, h->st, (int32_t)h->n, h->bs, (int32_t)h->nb, 0};
+        descs[i] = ArrayDesc{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0};
         for (int64_t off = 0; off < qs[i]; off += item_queries) {
             const int32_t c = (int32_t)std::min<int64_t>(item_queries, qs[i] - off);
             items.push_back(WorkItem{reinterpret_cast<const int2 *>(d_Qs[i]) + off,

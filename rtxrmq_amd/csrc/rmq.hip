@@ -80,7 +80,7 @@ __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__
 // answer_class.  The short-class kernel also writes (NaN, -1) for invalid
 // queries.  Persistent grid, static stride over tiles.
 template <int G, bool LONG>
-__global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A, int n, int bs, int nb,
+__global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A, int n, int bs, uint64_t magic, int nb,
                                                      const Pair *__restrict__ st, const int2 *__restrict__ Q,
                                                      int64_t q, int short_max, float *__restrict__ out_v,
                                                      int32_t *__restrict__ out_i) {
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
         const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
-        answer_class<G, LONG>(A, bs, nb, st, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
+        answer_class<G, LONG>(A, bs, magic, nb, st, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
                               stage[wv]);
     }
 }
@@ -189,11 +189,16 @@ template <bool LONG>
 int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int grid, int short_max, float *val,
                  int32_t *idx, hipStream_t s) {
     const int n = (int)h->n, bs = h->bs, nb = (int)h->nb;
+    const uint64_t mg = block_magic(bs);
     switch (lanes) {
-    case 4: k_query_tiles<4, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
-    case 8: k_query_tiles<8, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
-    case 16: k_query_tiles<16, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
-    default: k_query_tiles<64, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, nb, h->st, Q, q, short_max, val, idx); break;
+    case 4: k_query_tiles<4, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx); break;
+    case 8: k_query_tiles<8, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx); break;
+    case 16:
+        k_query_tiles<16, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx);
+        break;
+    default:
+        k_query_tiles<64, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx);
+        break;
     }
     return hip_rc(hipGetLastError());
 }
@@ -603,7 +608,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         rmq_handle_t *h = nullptr;
         if ((rc = rmq_build(d_A, n, bs, 0, s, &h)) != RMQ_OK) return rc;
         float ms = 0.f;
-        rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 2, &ms);
+        rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 3, &ms);
         rmq_free(h);
         if (rc != RMQ_OK) return rc;
         ++tried;
@@ -623,7 +628,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             o.short_max = sm;
             o.group_lanes = g;
             float ms = 0.f;
-            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 2, &ms)) != RMQ_OK) break;
+            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 3, &ms)) != RMQ_OK) break;
             ++tried;
             if (ms < best) {
                 best = ms;
@@ -638,7 +643,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         rmq_query_opts_t o = best_o;
         o.long_lanes = g;
         float ms = 0.f;
-        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 2, &ms)) != RMQ_OK) break;
+        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 3, &ms)) != RMQ_OK) break;
         ++tried;
         if (ms < best) {
             best = ms;

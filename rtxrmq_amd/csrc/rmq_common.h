@@ -172,14 +172,23 @@ __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64
 // between, the sparse table over block minima (:138-149) gives their leftmost
 // minimum in O(1).  The group's result is combined with the table's with the
 // index as tie-break, so the leftmost minimum wins.
+// Block index of element e: e / bs by one multiply-high with the handle's
+// magic = floor((2^64-1)/bs) + 1 (exact for 0 <= e < 2^31 and bs >= 2);
+// magic 0 means bs == 1.
+__host__ __device__ __forceinline__ uint64_t block_magic(int bs) { return bs > 1 ? UINT64_MAX / (uint64_t)bs + 1 : 0; }
+
+__device__ __forceinline__ int block_of(int e, uint64_t magic) {
+    return magic ? (int)__umul64hi((uint64_t)e, magic) : e;
+}
+
 template <int G, int U, bool LONG>
-__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, int nb,
+__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, uint64_t magic, int nb,
                                                const Pair *__restrict__ st, int l, int r, int gl, float &bv,
                                                int &bi, float &mv, int &mi) {
     if constexpr (!LONG) {
         scan_range<G, U>(A, l, r, gl, bv, bi);
     } else {
-        const int lB = l / bs, rB = r / bs;
+        const int lB = block_of(l, magic), rB = block_of(r, magic);
         if (lB == rB) {
             scan_range<G, U>(A, l, r, gl, bv, bi);
         } else {
@@ -212,7 +221,8 @@ struct TileStage {
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
 template <int G, bool LONG>
-__device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs, int nb, const Pair *__restrict__ st,
+__device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs, uint64_t magic, int nb,
+                                             const Pair *__restrict__ st,
                                              uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage) {
@@ -224,7 +234,7 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
             const int r = __builtin_amdgcn_readlane(m.r, src);
             float bv = INFINITY, mv = INFINITY;
             int bi = INT_MAX, mi = INT_MAX;
-            answer_partial<kWave, U, LONG>(A, bs, nb, st, l, r, lane, bv, bi, mv, mi);
+            answer_partial<kWave, U, LONG>(A, bs, magic, nb, st, l, r, lane, bv, bi, mv, mi);
             wave_reduce(bv, bi);
             take_if_better(mv, mi, bv, bi);
             if (lane == 0) {
@@ -253,7 +263,7 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
             int ql = 0;
             if (has) {
                 ql = stage.l[s];
-                answer_partial<G, U, LONG>(A, bs, nb, st, ql, stage.r[s], gl, bv, bi, mv, mi);
+                answer_partial<G, U, LONG>(A, bs, magic, nb, st, ql, stage.r[s], gl, bv, bi, mv, mi);
             }
             group_reduce<G>(bv, bi);
             take_if_better(mv, mi, bv, bi);

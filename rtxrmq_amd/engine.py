@@ -168,9 +168,12 @@ def check_queries(Q, n: int, stream=None) -> int:
     return bad.value
 
 
-def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, stream=None) -> dict:
+def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool = False, stream=None) -> dict:
     """Autotuner pick (rmq_tune) for this array and a sample of its queries.
 
+    With a cache file, a pick stored for the same (n, length signature) is
+    returned without timing anything; refresh=True times the sweep anyway and
+    appends the new pick (the last line for a key wins).
     Returns {"bs", "opts" (QueryOpts), "best_ms", "n_tried", "from_cache", "key"}.
     """
     torch = _torch()
@@ -180,11 +183,15 @@ def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, stream=None) 
     scratch = torch.empty(q, dtype=torch.int32, device=A.device)
     arr = (ctypes.c_int32 * len(bs_list))(*bs_list) if bs_list else None
     res = N.TuneResult()
+    use_cache = cache_path if (cache_path and not refresh) else None
     rc = N.lib().rmq_tune(A.data_ptr(), A.numel(), Q_sample.data_ptr(), q, arr, len(bs_list) if bs_list else 0,
-                          None, scratch.data_ptr(), cache_path.encode() if cache_path else None,
+                          None, scratch.data_ptr(), use_cache.encode() if use_cache else None,
                           _stream_ptr(stream), ctypes.byref(res))
     N.check(rc, "rmq_tune")
     o = res.opts
+    if cache_path and refresh:  # same line format rmq_tune writes
+        with open(cache_path, "a") as f:
+            f.write(f"{res.key.decode()} {res.bs} {o.short_max} {o.group_lanes} {o.long_lanes} {res.best_ms:.6f}\n")
     return {"bs": res.bs, "opts": N.QueryOpts(o.short_max, o.group_lanes, o.long_wgs, o.long_lanes),
             "best_ms": res.best_ms, "n_tried": res.n_tried, "from_cache": bool(res.from_cache),
             "key": res.key.decode()}
