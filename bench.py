@@ -263,6 +263,148 @@ def run_batched(R, torch, dev, args) -> dict:
             "steps": args.batch_steps, "identical_to_loop": identical}
 
 
+def run_served(R, torch, dev, args) -> dict:
+    """Config 5: resident arrays, requests of q log-uniform in [2^8, 2^22]
+    queries (1:1:1 uniform / lognormal 2^14-scaled / 2^6-scaled lengths, int32
+    indices, host buffers in pinned memory) through the continuous-batching
+    server: saturation throughput and a Poisson stream at a fixed offered
+    rate (p50/p99 latency), against the plain per-request loop (upload,
+    rmq_query, download, synchronise)."""
+    import threading
+
+    rng = np.random.default_rng(args.seed + 5)
+    m = args.serve_arrays
+    ns = np.floor(np.exp2(rng.uniform(24, 27, m))).astype(np.int64)
+    log(f"served: {m} resident arrays, {int(ns.sum())} elements")
+    arrays = [R.gen_array(int(n), args.seed * 31 + i, "uniform", device=dev) for i, n in enumerate(ns)]
+    big = int(np.argmax(ns))
+    sample = torch.cat([R.gen_queries_device(1 << 20, int(ns[big]), lr, 77, device=dev) for lr in (-1, -7, -8)])
+    pick = R.tune(arrays[big], sample, bs_list=[16, 32, 64, 128, 256, 1024], cache_path=args.tune_cache or None,
+                  refresh=args.retune)
+    bs, opts = pick["bs"], pick["opts"]
+    log(f"served: bs={bs} short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
+    # payload pool: per array, serve_payloads query batches in pinned host memory
+    payloads = []
+    for i, n in enumerate(ns):
+        for k in range(args.serve_payloads):
+            q = int(np.exp2(rng.uniform(8, 22)))
+            Q = torch.empty((q, 2), dtype=torch.int32, pin_memory=True)
+            parts = np.array_split(np.arange(q), 3)
+            Qn = Q.numpy()
+            for lr, part in zip((-1, -7, -8), parts):
+                if len(part):
+                    Qn[part[0]:part[-1] + 1] = R.gen_queries(len(part), int(n), lr, args.seed * 1009 + i * 97 + k)
+            payloads.append((i, Q))
+    pq = np.array([Q.shape[0] for _, Q in payloads])
+    log(f"served: {len(payloads)} payloads, mean q {pq.mean():.0f}")
+    max_q = int(pq.max())
+
+    # plain per-request loop on the same handles
+    indexes = [R.RMQIndex(A, bs) for A in arrays]
+    plain_out = {}
+    out_pin = torch.empty(max_q, dtype=torch.int32, pin_memory=True)
+    order = rng.integers(0, len(payloads), args.serve_plain_requests)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nq_plain = 0
+    for p_id in order:
+        a, Q = payloads[p_id]
+        Qd = Q.to(dev, non_blocking=True)
+        _, idx = indexes[a].query(Qd, want_value=False, want_index=True, opts=opts)
+        out_pin[:idx.numel()].copy_(idx, non_blocking=True)
+        torch.cuda.synchronize()
+        nq_plain += Q.shape[0]
+        if p_id not in plain_out:
+            plain_out[int(p_id)] = out_pin[:idx.numel()].clone()
+    plain_s = time.perf_counter() - t0
+    for ix in indexes:
+        ix.close()
+    plain_rate = nq_plain / plain_s
+    log(f"served: plain per-request loop {plain_rate:.4e} RMQ/s over {len(order)} requests")
+
+    sv = R.Server(slots=args.serve_slots, opts=opts)
+    ids = [sv.add_array(A, bs) for A in arrays]
+    ring = [torch.empty(max_q, dtype=torch.int32, pin_memory=True) for _ in range(args.serve_ring)]
+
+    def run_stream(n_req, rate_per_s, tag):
+        """Submits n_req random payloads (all at once when rate is None, else at
+        Poisson times); a waiter thread recycles output buffers."""
+        free = list(range(len(ring)))
+        cv = threading.Condition()
+        tickets = []
+        mismatches = [0]
+        done = threading.Event()
+        choice = rng.integers(0, len(payloads), n_req)
+
+        def waiter():
+            k = 0
+            while k < n_req:
+                with cv:
+                    while k >= len(tickets):
+                        cv.wait()
+                    tk, slot, p_id = tickets[k]
+                rc, _ = sv.wait(tk)
+                if rc != R.RMQ_OK:
+                    mismatches[0] += 1
+                elif p_id in plain_out:
+                    ref = plain_out[p_id]
+                    if not torch.equal(ring[slot][:ref.numel()], ref):
+                        mismatches[0] += 1
+                with cv:
+                    free.append(slot)
+                    cv.notify_all()
+                k += 1
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        sv.stats(reset=True)
+        gaps = rng.exponential(1.0 / rate_per_s, n_req) if rate_per_s else np.zeros(n_req)
+        t_next = time.perf_counter()
+        for k in range(n_req):
+            t_next += gaps[k]
+            while rate_per_s and time.perf_counter() < t_next:
+                pass
+            with cv:
+                while not free:
+                    cv.wait()
+                slot = free.pop()
+            p_id = int(choice[k])
+            a, Q = payloads[p_id]
+            tk = sv.submit(ids[a], Q, out_idx=ring[slot][:Q.shape[0]])
+            with cv:
+                tickets.append((tk, slot, p_id))
+                cv.notify_all()
+        done.wait()
+        th.join()
+        st = sv.stats()
+        rate = st["queries_done"] / (st["wall_ms"] / 1e3)
+        log(f"served[{tag}]: {st['requests_done']} requests, {rate:.4e} RMQ/s, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, {st['mean_requests_per_launch']:.1f} requests/launch, "
+            f"launch: enqueue {st['mean_enqueue_ms']:.2f} ms, device {st['mean_device_ms']:.2f} ms, "
+            f"wait {st['mean_launch_wait_ms']:.2f} ms; mismatches {mismatches[0]}")
+        return st, rate, mismatches[0]
+
+    run_stream(64, None, "warmup")
+    sat, sat_rate, sat_bad = run_stream(args.serve_requests, None, "saturation")
+    req_rate = sat["requests_done"] / (sat["wall_ms"] / 1e3)
+    offered = args.serve_load * req_rate
+    poi, poi_rate, poi_bad = run_stream(args.serve_requests, offered, f"poisson {offered:.0f} req/s")
+    sv.close()
+    return {"arrays": m, "elements": int(ns.sum()), "bs": bs, "slots": args.serve_slots,
+            "request_q": "log-uniform [2^8, 2^22]", "mean_q": float(pq.mean()), "payloads": len(payloads),
+            "plain_rmq_per_s": plain_rate, "plain_requests": len(order),
+            "saturation": {"rmq_per_s": sat_rate, "requests": sat["requests_done"], "p50_ms": sat["p50_ms"],
+                           "p99_ms": sat["p99_ms"], "requests_per_launch": sat["mean_requests_per_launch"]},
+            "poisson": {"offered_requests_per_s": offered, "offered_fraction_of_saturation": args.serve_load,
+                        "rmq_per_s": poi_rate, "requests": poi["requests_done"], "p50_ms": poi["p50_ms"],
+                        "p99_ms": poi["p99_ms"], "mean_ms": poi["mean_ms"], "max_ms": poi["max_ms"],
+                        "requests_per_launch": poi["mean_requests_per_launch"]},
+            "speedup_vs_plain": sat_rate / plain_rate, "mismatches_vs_plain": sat_bad + poi_bad,
+            "budget_bytes": sat["budget_bytes"], "peak_inflight_bytes": max(sat["peak_inflight_bytes"],
+                                                                            poi["peak_inflight_bytes"])}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -283,6 +425,16 @@ def main() -> int:
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-arrays", type=int, default=16)
+    ap.add_argument("--serve-payloads", type=int, default=8, help="query batches per resident array")
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--serve-plain-requests", type=int, default=300)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-ring", type=int, default=96, help="pinned output buffers in flight")
+    ap.add_argument("--serve-load", type=float, default=0.7, help="offered rate / saturation rate")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--only", choices=["batched", "served"], default=None,
+                    help="run one section alone and print its JSON object (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -308,6 +460,10 @@ def main() -> int:
     import rtxrmq_amd as R
 
     dev = torch.device("cuda:0")
+    if args.only:
+        res = run_batched(R, torch, dev, args) if args.only == "batched" else run_served(R, torch, dev, args)
+        print(json.dumps({args.only: res}), flush=True)
+        return 0
     n, q = args.n, args.q
     log(f"bench: n={n} q={q} dists={DISTS} steps={args.steps} warmup={args.warmup}")
 
@@ -364,6 +520,7 @@ def main() -> int:
     workload = f"n={n} q={q} lr=-1,-2,-3 value+index tuned"
     del out_v, out_i
     batched = None if args.no_batched else run_batched(R, torch, dev, args)
+    served = None if args.no_served else run_served(R, torch, dev, args)
     cpu = None
     if not args.no_cpu_baseline:
         A_host = A.cpu().numpy()
@@ -395,6 +552,7 @@ def main() -> int:
                      "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"]},
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
         "per_dist": tuned["per_dist"],
         "per_kernel": tuned["per_kernel"],
         f"fixed_bs_{args.bs}": {

@@ -136,7 +136,8 @@ int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const i
  *      serving loop) answered by ONE persistent launch.  No counterpart in the
  *      reference, which answers one array per run (src/main.cu:51-171).
  *      A plan holds, on the device, one descriptor per handle and a work list
- *      of chunks of at most item_queries (0: 4096) queries; array i's d_Qs[i]
+ *      of chunks of at most item_queries queries (0: sized so the launch has
+ *      about four items per resident wave, 64..4096); array i's d_Qs[i]
  *      holds qs[i] pairs {l, r} (0 <= l <= r < n_i) and its answers go to
  *      d_vals[i] and/or d_idxs[i] (per-array NULL allowed; d_vals or d_idxs
  *      may be NULL as a whole).  opts applies to every array (NULL =
@@ -158,6 +159,57 @@ void rmq_plan_free(rmq_plan_t *p);
  * each array in turn, same arguments as rmq_plan_create. */
 int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs, const int64_t *qs,
                    float *const *d_vals, int32_t *const *d_idxs, const rmq_query_opts_t *opts, hipStream_t s);
+
+/* ---- serving loop (no counterpart in the reference): continuous batching of
+ *      query requests over arrays kept resident in HBM.
+ *      - rmq_server_add_array builds a handle for a caller-owned device array
+ *        (kept alive by the caller) after checking that its block table fits
+ *        in free HBM beside the request budget; returns its id.
+ *      - rmq_server_submit queues a request: q host query pairs for one array
+ *        and host output buffers (pinned memory gives asynchronous copies);
+ *        it returns at once with a ticket.  The buffers must stay valid until
+ *        rmq_server_wait returns for that ticket.
+ *      - A scheduler thread admits pending requests FIFO while their device
+ *        buffers fit in the memory budget (free HBM at creation less a
+ *        reserve, or mem_budget_bytes if smaller) and coalesces them, up to
+ *        max_batch_queries, into one plan launch on a free slot (stream):
+ *        upload, one persistent kernel over all their work items, download.
+ *        Up to `slots` launches are in flight; requests arriving meanwhile go
+ *        to the next free slot, so their work overlaps on the device.
+ *      - Latency is measured from submit to the answers in host memory.
+ *      Answers are the same as rmq_query's for the same array and queries,
+ *      whatever else shares the launch. ---- */
+typedef struct rmq_server rmq_server_t;
+typedef struct rmq_server_config {
+    int32_t slots;             /* launches in flight at once (0: 3)                    */
+    int32_t max_arrays;        /* resident arrays (0: 1024)                            */
+    int64_t mem_budget_bytes;  /* cap on request device memory in flight (0: from HBM) */
+    int64_t max_batch_queries; /* queries coalesced into one launch (0: 2^24)          */
+    int32_t item_queries;      /* queries per work item (0: sized per launch, 64..4096) */
+    int32_t reserved;
+    rmq_query_opts_t opts;     /* applied to every array (zeros: defaults)             */
+} rmq_server_config_t;
+typedef struct rmq_server_stats {
+    int64_t requests_done, queries_done, launches, failed;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> answers in host memory */
+    double wall_ms;                         /* first submit -> last completion  */
+    double mean_requests_per_launch;
+    int64_t budget_bytes, peak_inflight_bytes;
+    double mean_enqueue_ms;     /* host time to enqueue one launch            */
+    double mean_device_ms;      /* device time of one launch (HIP events)     */
+    double mean_launch_wait_ms; /* end of enqueue -> completion noticed       */
+} rmq_server_stats_t;
+int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out);
+int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, int32_t *array_id);
+int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, int64_t q, int32_t *h_idx,
+                      float *h_val, uint64_t *ticket);
+/* Blocks until the request is answered; returns its status (RMQ_ERR_OOM if it
+ * can never fit the budget) and forgets the ticket. */
+int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_ms);
+/* Latency percentiles and throughput since the last reset (reset != 0 clears). */
+int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset);
+/* Finishes the launches in flight, then frees everything (not the arrays). */
+void rmq_server_destroy(rmq_server_t *sv);
 
 /* ---- brute force on the device, algs 2/102 and the --check oracle:
  *      replaces kernel_rmq_basic / kernel_rmq_basic_idx / gpu_rmq_basic

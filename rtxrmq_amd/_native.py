@@ -54,6 +54,22 @@ class TuneResult(ctypes.Structure):
                 ("from_cache", _i32), ("key", ctypes.c_char * 96)]
 
 
+class ServerConfig(ctypes.Structure):
+    """rmq_server_config_t"""
+    _fields_ = [("slots", _i32), ("max_arrays", _i32), ("mem_budget_bytes", _i64), ("max_batch_queries", _i64),
+                ("item_queries", _i32), ("reserved", _i32), ("opts", QueryOpts)]
+
+
+class ServerStats(ctypes.Structure):
+    """rmq_server_stats_t"""
+    _fields_ = [("requests_done", _i64), ("queries_done", _i64), ("launches", _i64), ("failed", _i64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("mean_requests_per_launch", ctypes.c_double), ("budget_bytes", _i64),
+                ("peak_inflight_bytes", _i64), ("mean_enqueue_ms", ctypes.c_double),
+                ("mean_device_ms", ctypes.c_double), ("mean_launch_wait_ms", ctypes.c_double)]
+
+
 _LIB = None
 
 
@@ -83,6 +99,12 @@ def _declare(lib):
         "rmq_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
         "rmq_plan_free": (None, [_vp]),
         "rmq_query_loop": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _vp]),
+        "rmq_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+        "rmq_server_add_array": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.POINTER(_i32)]),
+        "rmq_server_submit": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, ctypes.POINTER(_u64)]),
+        "rmq_server_wait": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
+        "rmq_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats), _i32]),
+        "rmq_server_destroy": (None, [_vp]),
         "rmq_strerror": (ctypes.c_char_p, [ctypes.c_int]),
         "rmq_version": (ctypes.c_char_p, []),
     }

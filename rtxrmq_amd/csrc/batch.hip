@@ -71,6 +71,16 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
 
 using PlanKernel = void (*)(const ArrayDesc *, const WorkItem *, int, unsigned *, int);
 
+// Queries per work item when the caller leaves it to us: about four items per
+// resident wave (8 workgroups of 4 waves per CU), so a small launch still
+// fills the GPU, clamped to [64, 4096] and a multiple of the 64-query tile.
+inline int32_t auto_item_queries(int64_t total_queries) {
+    const int64_t target_items = (int64_t)device_cus() * 8 * kWavesPerWG * 4;
+    int64_t per = (total_queries + target_items - 1) / target_items;
+    per = (per + kWave - 1) / kWave * kWave;
+    return (int32_t)std::max<int64_t>(kWave, std::min<int64_t>(4096, per));
+}
+
 template <int GS>
 PlanKernel pick_long(int gl) {
     switch (gl) {
@@ -140,7 +150,11 @@ RMQ_API int rmq_plan_create(const rmq_handle_t *const *handles, int32_t m, const
     if (!out) return RMQ_ERR_INVALID_ARG;
     *out = nullptr;
     if (m < 0 || (m > 0 && (!handles || !d_Qs || !qs)) || item_queries < 0) return RMQ_ERR_INVALID_ARG;
-    if (item_queries == 0) item_queries = 4096;
+    if (item_queries == 0) {
+        int64_t total = 0;
+        for (int32_t i = 0; i < m; ++i) total += std::max<int64_t>(0, qs[i]);
+        item_queries = auto_item_queries(total);
+    }
     rmq_plan *p = new (std::nothrow) rmq_plan{};
     if (!p) return RMQ_ERR_OOM;
     int rc = hip_rc(hipGetDevice(&p->device));
@@ -267,4 +281,472 @@ RMQ_API int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const 
         if (rc != RMQ_OK) return rc;
     }
     return RMQ_OK;
+}
+
+// ===========================================================================
+// serving loop: continuous batching over resident arrays
+// ===========================================================================
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int32_t array = 0;
+    const int32_t *h_Q = nullptr;
+    int64_t q = 0;
+    int32_t *h_idx = nullptr;
+    float *h_val = nullptr;
+    int64_t bytes = 0;  // device bytes while in flight
+    double t_submit = 0, t_done = 0;
+    int status = RMQ_OK;
+    bool done = false;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t start_ev = nullptr;
+    hipEvent_t done_ev = nullptr;
+    double t_enqueued = 0;  // host time launch_group returned
+    WorkItem *h_items = nullptr;  // pinned staging of the work list
+    WorkItem *d_items = nullptr;
+    size_t items_cap = 0;
+    unsigned *d_head = nullptr;
+    void *d_buf = nullptr;
+    int64_t bytes = 0;
+    bool busy = false;      // a launch is in flight
+    bool launched = false;  // its completion is awaited
+    std::vector<uint64_t> tickets;
+};
+
+int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
+
+// Device bytes a request holds while in flight: its queries and outputs,
+// each rounded to 256 bytes (the carving launch_group does).
+int64_t request_bytes(const Request &r) {
+    return round256(r.q * 8) + (r.h_idx ? round256(r.q * 4) : 0) + (r.h_val ? round256(r.q * 4) : 0);
+}
+
+}  // namespace
+
+struct rmq_server {
+    rmq_server_config_t cfg{};
+    int device = -1;
+    std::vector<rmq_handle *> arrays;
+    ArrayDesc *d_descs = nullptr;
+    int32_t descs_cap = 0;
+    PlanKernel kernel = nullptr;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done, cv_slot;
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, Request> reqs;
+    uint64_t next_ticket = 1;
+    int64_t budget = 0, inflight = 0, peak_inflight = 0;
+    bool stop = false;
+    std::thread sched;
+    std::vector<std::thread> completers;
+    // statistics
+    std::vector<double> lat_ms;
+    int64_t requests_done = 0, queries_done = 0, launches = 0, failed = 0, batched_requests = 0;
+    double first_submit = -1, last_done = -1;
+    double sum_enqueue_ms = 0, sum_device_ms = 0, sum_wait_ms = 0;
+};
+
+namespace {
+
+int launch_group(rmq_server *sv, Slot &sl, std::vector<Request *> &group) {
+    int64_t total = 0, total_q = 0;
+    for (Request *r : group) {
+        total += r->bytes;
+        total_q += r->q;
+    }
+    const int iq = sv->cfg.item_queries > 0 ? sv->cfg.item_queries : auto_item_queries(total_q);
+    size_t n_items = 0;
+    for (Request *r : group) n_items += (size_t)((r->q + iq - 1) / iq);
+    if (n_items > sl.items_cap) {  // the slot is idle: its previous launch completed
+        if (sl.h_items) (void)hipHostFree(sl.h_items);
+        if (sl.d_items) (void)hipFree(sl.d_items);
+        sl.h_items = nullptr;
+        sl.d_items = nullptr;
+        sl.items_cap = 0;
+        const size_t cap = std::max<size_t>(n_items, 1024);
+        int rc = hip_rc(hipHostMalloc((void **)&sl.h_items, sizeof(WorkItem) * cap, hipHostMallocDefault));
+        if (rc == RMQ_OK) rc = hip_rc(hipMalloc(&sl.d_items, sizeof(WorkItem) * cap));
+        if (rc != RMQ_OK) return rc;
+        sl.items_cap = cap;
+    }
+    int rc = hip_rc(hipEventRecord(sl.start_ev, sl.stream));
+    if (rc == RMQ_OK) rc = hip_rc(hipMallocAsync(&sl.d_buf, (size_t)total, sl.stream));
+    if (rc != RMQ_OK) return rc;
+    char *p = static_cast<char *>(sl.d_buf);
+    size_t k = 0;
+    for (Request *r : group) {
+        int32_t *d_Q = reinterpret_cast<int32_t *>(p);
+        p += round256(r->q * 8);
+        int32_t *d_idx = nullptr;
+        float *d_val = nullptr;
+        if (r->h_idx) {
+            d_idx = reinterpret_cast<int32_t *>(p);
+            p += round256(r->q * 4);
+        }
+        if (r->h_val) {
+            d_val = reinterpret_cast<float *>(p);
+            p += round256(r->q * 4);
+        }
+        if (rc == RMQ_OK)
+            rc = hip_rc(hipMemcpyAsync(d_Q, r->h_Q, (size_t)r->q * 8, hipMemcpyHostToDevice, sl.stream));
+        for (int64_t off = 0; off < r->q; off += iq) {
+            sl.h_items[k++] = WorkItem{reinterpret_cast<const int2 *>(d_Q) + off, d_val ? d_val + off : nullptr,
+                                       d_idx ? d_idx + off : nullptr, (int32_t)std::min<int64_t>(iq, r->q - off),
+                                       r->array};
+        }
+    }
+    if (rc == RMQ_OK)
+        rc = hip_rc(hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(WorkItem) * n_items, hipMemcpyHostToDevice,
+                                   sl.stream));
+    if (rc == RMQ_OK) {
+        k_reset_head<<<1, 64, 0, sl.stream>>>(sl.d_head);
+        const int grid = (int)std::max<int64_t>(
+            1, std::min<int64_t>(((int64_t)n_items + kWavesPerWG - 1) / kWavesPerWG, (int64_t)device_cus() * 8));
+        hipLaunchKernelGGL(sv->kernel, dim3(grid), dim3(kWG), 0, sl.stream, sv->d_descs, sl.d_items, (int)n_items,
+                           sl.d_head, sv->cfg.opts.short_max);
+        rc = hip_rc(hipGetLastError());
+    }
+    // copy the answers back, request by request
+    p = static_cast<char *>(sl.d_buf);
+    for (Request *r : group) {
+        p += round256(r->q * 8);
+        if (r->h_idx) {
+            if (rc == RMQ_OK)
+                rc = hip_rc(hipMemcpyAsync(r->h_idx, p, (size_t)r->q * 4, hipMemcpyDeviceToHost, sl.stream));
+            p += round256(r->q * 4);
+        }
+        if (r->h_val) {
+            if (rc == RMQ_OK)
+                rc = hip_rc(hipMemcpyAsync(r->h_val, p, (size_t)r->q * 4, hipMemcpyDeviceToHost, sl.stream));
+            p += round256(r->q * 4);
+        }
+    }
+    const int rc_free = hip_rc(hipFreeAsync(sl.d_buf, sl.stream));
+    sl.d_buf = nullptr;
+    const int rc_ev = hip_rc(hipEventRecord(sl.done_ev, sl.stream));
+    return rc != RMQ_OK ? rc : (rc_free != RMQ_OK ? rc_free : rc_ev);
+}
+
+void scheduler_loop(rmq_server *sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    const int64_t max_q = sv->cfg.max_batch_queries > 0 ? sv->cfg.max_batch_queries : (int64_t)1 << 24;
+    for (;;) {
+        Slot *free_slot = nullptr;
+        sv->cv_sched.wait(lk, [&] {
+            if (sv->stop) return true;
+            if (sv->pending.empty()) return false;
+            for (Slot &s : sv->slots)
+                if (!s.busy) {
+                    free_slot = &s;
+                    return true;
+                }
+            return false;
+        });
+        if (sv->stop) return;
+        // admission: FIFO, as many pending requests as the memory budget and
+        // the per-launch query cap allow
+        std::vector<Request *> group;
+        std::vector<uint64_t> tickets;
+        int64_t bytes = 0, nq = 0;
+        while (!sv->pending.empty()) {
+            Request &r = sv->reqs[sv->pending.front()];
+            if (r.bytes > sv->budget) {  // can never fit: fail it
+                r.status = RMQ_ERR_OOM;
+                r.done = true;
+                r.t_done = now_ms();
+                ++sv->failed;
+                sv->pending.pop_front();
+                sv->cv_done.notify_all();
+                continue;
+            }
+            if (sv->inflight + bytes + r.bytes > sv->budget) break;
+            if (!group.empty() && nq + r.q > max_q) break;
+            group.push_back(&r);
+            tickets.push_back(sv->pending.front());
+            bytes += r.bytes;
+            nq += r.q;
+            sv->pending.pop_front();
+        }
+        if (group.empty()) {
+            // waiting for memory: sleep until a launch completes
+            sv->cv_sched.wait(lk, [&] {
+                if (sv->stop || sv->pending.empty()) return true;
+                return sv->inflight + sv->reqs[sv->pending.front()].bytes <= sv->budget;
+            });
+            continue;
+        }
+        sv->inflight += bytes;
+        sv->peak_inflight = std::max(sv->peak_inflight, sv->inflight);
+        free_slot->busy = true;
+        free_slot->bytes = bytes;
+        free_slot->tickets = tickets;
+        ++sv->launches;
+        sv->batched_requests += (int64_t)group.size();
+        lk.unlock();
+        const double t0 = now_ms();
+        const int rc = launch_group(sv, *free_slot, group);
+        free_slot->t_enqueued = now_ms();
+        lk.lock();
+        sv->sum_enqueue_ms += free_slot->t_enqueued - t0;
+        if (rc != RMQ_OK)
+            for (Request *r : group) r->status = rc;
+        free_slot->launched = true;
+        sv->cv_slot.notify_all();
+    }
+}
+
+void completer_loop(rmq_server *sv, size_t si) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    Slot &sl = sv->slots[si];
+    for (;;) {
+        sv->cv_slot.wait(lk, [&] { return sv->stop || sl.launched; });
+        if (sl.launched) {
+            lk.unlock();
+            // poll with a short sleep: completion is noticed within tens of
+            // microseconds without a core spinning on the event
+            hipError_t e;
+            while ((e = hipEventQuery(sl.done_ev)) == hipErrorNotReady)
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            const double t = now_ms();
+            float dev_ms = 0.f;
+            if (e == hipSuccess) (void)hipEventElapsedTime(&dev_ms, sl.start_ev, sl.done_ev);
+            lk.lock();
+            sv->sum_device_ms += dev_ms;
+            sv->sum_wait_ms += t - sl.t_enqueued;
+            for (uint64_t tk : sl.tickets) {
+                Request &r = sv->reqs[tk];
+                if (e != hipSuccess && r.status == RMQ_OK) r.status = RMQ_ERR_HIP;
+                r.done = true;
+                r.t_done = t;
+                if (r.status == RMQ_OK) {
+                    ++sv->requests_done;
+                    sv->queries_done += r.q;
+                    sv->lat_ms.push_back(t - r.t_submit);
+                } else {
+                    ++sv->failed;
+                }
+            }
+            sv->last_done = t;
+            sv->inflight -= sl.bytes;
+            sl.bytes = 0;
+            sl.tickets.clear();
+            sl.launched = false;
+            sl.busy = false;
+            sv->cv_done.notify_all();
+            sv->cv_sched.notify_all();
+            continue;
+        }
+        if (sv->stop) return;
+    }
+}
+
+void server_release(rmq_server *sv) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_sched.notify_all();
+    sv->cv_slot.notify_all();
+    if (sv->sched.joinable()) sv->sched.join();
+    for (auto &t : sv->completers)
+        if (t.joinable()) t.join();
+    for (Slot &s : sv->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        if (s.done_ev) (void)hipEventDestroy(s.done_ev);
+        if (s.start_ev) (void)hipEventDestroy(s.start_ev);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+        if (s.h_items) (void)hipHostFree(s.h_items);
+        if (s.d_items) (void)hipFree(s.d_items);
+        if (s.d_head) (void)hipFree(s.d_head);
+    }
+    for (rmq_handle *h : sv->arrays) rmq_free(h);
+    if (sv->d_descs) (void)hipFree(sv->d_descs);
+    delete sv;
+}
+
+}  // namespace
+
+RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out) {
+    if (!out) return RMQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    rmq_server_config_t c{};
+    if (cfg) c = *cfg;
+    if (c.slots < 0 || c.max_arrays < 0 || c.mem_budget_bytes < 0 || c.max_batch_queries < 0 || c.item_queries < 0)
+        return RMQ_ERR_INVALID_ARG;
+    if (c.slots == 0) c.slots = 3;
+    if (c.max_arrays == 0) c.max_arrays = 1024;
+    rmq_server *sv = new (std::nothrow) rmq_server{};
+    if (!sv) return RMQ_ERR_OOM;
+    int rc = hip_rc(hipGetDevice(&sv->device));
+    rmq_handle dummy{};
+    if (rc == RMQ_OK) rc = resolve_opts(&dummy, &c.opts, &c.opts);
+    sv->cfg = c;
+    if (rc == RMQ_OK) {
+        // request buffers come from the device's stream-ordered pool; keep what
+        // is freed cached instead of returning it at every synchronisation
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, sv->device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
+    size_t free_b = 0, total_b = 0;
+    if (rc == RMQ_OK) rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
+    if (rc == RMQ_OK) {
+        // admission budget for request buffers: free HBM less a reserve for
+        // array tables and the runtime, or the configured cap if smaller
+        const int64_t reserve = (int64_t)std::min<size_t>(free_b / 4, (size_t)8 << 30);
+        sv->budget = (int64_t)free_b - reserve;
+        if (c.mem_budget_bytes > 0) sv->budget = std::min<int64_t>(sv->budget, c.mem_budget_bytes);
+        sv->kernel = pick_kernel(c.opts.group_lanes, c.opts.long_lanes);
+        rc = hip_rc(hipMalloc(&sv->d_descs, sizeof(ArrayDesc) * (size_t)c.max_arrays));
+        sv->descs_cap = c.max_arrays;
+    }
+    sv->slots.resize((size_t)c.slots);
+    for (Slot &s : sv->slots) {
+        if (rc == RMQ_OK) rc = hip_rc(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        if (rc == RMQ_OK) rc = hip_rc(hipEventCreate(&s.start_ev));
+        if (rc == RMQ_OK) rc = hip_rc(hipEventCreate(&s.done_ev));
+        if (rc == RMQ_OK) rc = hip_rc(hipMalloc(&s.d_head, 256));
+    }
+    if (rc != RMQ_OK) {
+        server_release(sv);
+        return rc;
+    }
+    sv->sched = std::thread(scheduler_loop, sv);
+    for (size_t i = 0; i < sv->slots.size(); ++i) sv->completers.emplace_back(completer_loop, sv, i);
+    *out = sv;
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, int32_t *array_id) {
+    if (!sv || !array_id || !d_A || n <= 0 || n > INT32_MAX || bs <= 0) return RMQ_ERR_INVALID_ARG;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
+    if (dev != sv->device) return RMQ_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if ((int32_t)sv->arrays.size() >= sv->descs_cap) return RMQ_ERR_OOM;
+    // admission of a new resident array: its block table must fit in free HBM
+    // beside the request budget still available
+    const int64_t nb = (n + bs - 1) / bs;
+    int64_t levels = 0;
+    while (((int64_t)1 << levels) <= nb) ++levels;
+    const int64_t table = (int64_t)sizeof(Pair) * nb * levels;
+    size_t free_b = 0, total_b = 0;
+    int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
+    if (rc != RMQ_OK) return rc;
+    // the memory promised to request buffers (budget not yet in flight) stays free
+    if (table + (sv->budget - sv->inflight) > (int64_t)free_b) return RMQ_ERR_OOM;
+    rmq_handle_t *h = nullptr;
+    hipStream_t s = sv->slots[0].stream;
+    rc = rmq_build(d_A, n, bs, 0, s, &h);
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    if (rc != RMQ_OK) {
+        rmq_free(h);
+        return rc;
+    }
+    const ArrayDesc d{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0};
+    const int32_t id = (int32_t)sv->arrays.size();
+    // a new entry is written beyond every entry an in-flight launch can read
+    rc = hip_rc(hipMemcpy(sv->d_descs + id, &d, sizeof(d), hipMemcpyHostToDevice));
+    if (rc != RMQ_OK) {
+        rmq_free(h);
+        return rc;
+    }
+    sv->arrays.push_back(h);
+    *array_id = id;
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, int64_t q, int32_t *h_idx,
+                              float *h_val, uint64_t *ticket) {
+    if (!sv || !ticket || q <= 0 || q > INT32_MAX || !h_Q || (!h_idx && !h_val)) return RMQ_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (array_id < 0 || array_id >= (int32_t)sv->arrays.size()) return RMQ_ERR_INVALID_ARG;
+    Request r;
+    r.array = array_id;
+    r.h_Q = h_Q;
+    r.q = q;
+    r.h_idx = h_idx;
+    r.h_val = h_val;
+    r.bytes = request_bytes(r);
+    r.t_submit = now_ms();
+    if (sv->first_submit < 0) sv->first_submit = r.t_submit;
+    const uint64_t tk = sv->next_ticket++;
+    sv->reqs.emplace(tk, r);
+    sv->pending.push_back(tk);
+    *ticket = tk;
+    sv->cv_sched.notify_one();
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_ms) {
+    if (!sv) return RMQ_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return RMQ_ERR_INVALID_ARG;
+    sv->cv_done.wait(lk, [&] { return it->second.done; });
+    const int status = it->second.status;
+    if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
+    sv->reqs.erase(it);
+    return status;
+}
+
+RMQ_API int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset) {
+    if (!sv || !st) return RMQ_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    *st = rmq_server_stats_t{};
+    st->requests_done = sv->requests_done;
+    st->queries_done = sv->queries_done;
+    st->launches = sv->launches;
+    st->failed = sv->failed;
+    st->budget_bytes = sv->budget;
+    st->peak_inflight_bytes = sv->peak_inflight;
+    st->mean_requests_per_launch = sv->launches ? (double)sv->batched_requests / sv->launches : 0.0;
+    if (!sv->lat_ms.empty()) {
+        std::vector<double> v = sv->lat_ms;
+        std::sort(v.begin(), v.end());
+        auto at = [&](double p) { return v[(size_t)std::min<double>(v.size() - 1, p * (v.size() - 1) + 0.5)]; };
+        st->p50_ms = at(0.50);
+        st->p99_ms = at(0.99);
+        st->max_ms = v.back();
+        double sum = 0;
+        for (double x : v) sum += x;
+        st->mean_ms = sum / v.size();
+    }
+    st->wall_ms = (sv->first_submit >= 0 && sv->last_done >= 0) ? sv->last_done - sv->first_submit : 0.0;
+    if (sv->launches) {
+        st->mean_enqueue_ms = sv->sum_enqueue_ms / sv->launches;
+        st->mean_device_ms = sv->sum_device_ms / sv->launches;
+        st->mean_launch_wait_ms = sv->sum_wait_ms / sv->launches;
+    }
+    if (reset) {
+        sv->lat_ms.clear();
+        sv->requests_done = sv->queries_done = sv->launches = sv->failed = sv->batched_requests = 0;
+        sv->first_submit = sv->last_done = -1;
+        sv->peak_inflight = sv->inflight;
+        sv->sum_enqueue_ms = sv->sum_device_ms = sv->sum_wait_ms = 0;
+    }
+    return RMQ_OK;
+}
+
+RMQ_API void rmq_server_destroy(rmq_server_t *sv) {
+    if (sv) server_release(sv);
 }

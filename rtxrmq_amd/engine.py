@@ -268,6 +268,86 @@ class Plan:
         self.close()
 
 
+class Server:
+    """Continuous-batching request loop over resident arrays (rmq_server_*).
+
+    add_array keeps the array tensor referenced; submit keeps the request's
+    host buffers referenced until wait returns.  Host buffers may be numpy
+    arrays or (for asynchronous copies) pinned torch tensors.
+    """
+
+    def __init__(self, slots: int = 0, mem_budget_bytes: int = 0, max_batch_queries: int = 0,
+                 item_queries: int = 0, opts: N.QueryOpts | None = None):
+        cfg = N.ServerConfig(slots, 0, mem_budget_bytes, max_batch_queries, item_queries, 0,
+                             opts if opts is not None else N.QueryOpts())
+        h = ctypes.c_void_p()
+        N.check(N.lib().rmq_server_create(ctypes.byref(cfg), ctypes.byref(h)), "rmq_server_create")
+        self._h = h
+        self._arrays = []
+        self._live = {}
+
+    def add_array(self, A, bs: int) -> int:
+        A = _dev_f32(A, "A")
+        aid = ctypes.c_int32()
+        N.check(N.lib().rmq_server_add_array(self._h, A.data_ptr(), A.numel(), bs, ctypes.byref(aid)),
+                "rmq_server_add_array")
+        self._arrays.append(A)
+        return aid.value
+
+    @staticmethod
+    def _host_ptr(x):
+        if x is None:
+            return None, None
+        if isinstance(x, np.ndarray):
+            if not x.flags["C_CONTIGUOUS"]:
+                raise TypeError("host buffers must be contiguous")
+            return x, x.ctypes.data
+        if x.device.type != "cpu" or not x.is_contiguous():
+            raise TypeError("host buffers must be contiguous CPU tensors")
+        return x, x.data_ptr()
+
+    def submit(self, array_id: int, Q, out_idx=None, out_val=None) -> int:
+        """Q: (q, 2) int32 host queries; out_idx / out_val: host outputs of q entries."""
+        q = (Q.size if isinstance(Q, np.ndarray) else Q.numel()) // 2
+        keep_q, pq = self._host_ptr(Q)
+        keep_i, pi = self._host_ptr(out_idx)
+        keep_v, pv = self._host_ptr(out_val)
+        tk = ctypes.c_uint64()
+        N.check(N.lib().rmq_server_submit(self._h, array_id, pq, q, pi, pv, ctypes.byref(tk)), "rmq_server_submit")
+        self._live[tk.value] = (keep_q, keep_i, keep_v)
+        return tk.value
+
+    def wait(self, ticket: int) -> tuple[int, float]:
+        """(status, latency_ms); status is RMQ_OK or a negative code."""
+        lat = ctypes.c_double()
+        rc = N.lib().rmq_server_wait(self._h, ticket, ctypes.byref(lat))
+        self._live.pop(ticket, None)
+        return rc, lat.value
+
+    def stats(self, reset: bool = False) -> dict:
+        st = N.ServerStats()
+        N.check(N.lib().rmq_server_stats(self._h, ctypes.byref(st), 1 if reset else 0), "rmq_server_stats")
+        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().rmq_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def gen_array_host(n: int, seed: int, kind: str = "uniform") -> np.ndarray:
     A = np.empty(n, np.float32)
     N.check(N.lib().rmq_gen_array_host(A.ctypes.data, n, seed, N.GEN_KINDS[kind]), "rmq_gen_array_host")

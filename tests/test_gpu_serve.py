@@ -1,0 +1,104 @@
+"""Serving loop: continuous batching over resident arrays (rmq_server_*).
+
+Each answer must be the same alone and in the mix, and equal to the CPU
+oracle's (tolerance 0: exact value bits and leftmost index).  Admission by
+device memory is exercised with a small budget: requests wait for memory
+instead of failing, a request that can never fit fails cleanly, and nothing
+else is disturbed.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import rtxrmq_amd as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def arrays(gpu):
+    out = []
+    for i, (n, kind) in enumerate([(1 << 16, "uniform"), (100_003, "ties16"), (1 << 20, "uniform"),
+                                   (777, "ties16")]):
+        A = O.gen_array(n, 40 + i, kind)
+        out.append((A, torch.from_numpy(A).to(gpu)))
+    return out
+
+
+def request_payloads(arrays, count, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        a = int(rng.integers(0, len(arrays)))
+        n = arrays[a][0].size
+        q = int(np.exp2(rng.uniform(4, 15)))
+        lr = [-1, -7, -8][k % 3]
+        reqs.append((a, O.gen_queries(q, n, lr, seed * 1000 + k)))
+    return reqs
+
+
+def test_alone_and_in_the_mix(arrays, gpu):
+    with R.Server(slots=3) as sv:
+        ids = [sv.add_array(Ad, bs) for (_, Ad), bs in zip(arrays, (64, 1000, 256, 3))]
+        reqs = request_payloads(arrays, 48, 5)
+        # request 0 alone
+        a0, Q0 = reqs[0]
+        alone = np.empty(Q0.shape[0], np.int32)
+        alone_v = np.empty(Q0.shape[0], np.float32)
+        tk = sv.submit(ids[a0], Q0, out_idx=alone, out_val=alone_v)
+        assert sv.wait(tk)[0] == R.RMQ_OK
+        # everything at once, request 0 included again
+        outs, tickets = [], []
+        for a, Q in reqs:
+            oi = np.full(Q.shape[0], -5, np.int32)
+            ov = np.zeros(Q.shape[0], np.float32)
+            outs.append((oi, ov))
+            tickets.append(sv.submit(ids[a], Q, out_idx=oi, out_val=ov))
+        lats = []
+        for tk in tickets:
+            rc, lat = sv.wait(tk)
+            assert rc == R.RMQ_OK
+            lats.append(lat)
+        st = sv.stats()
+    np.testing.assert_array_equal(outs[0][0], alone)
+    np.testing.assert_array_equal(outs[0][1].view(np.int32), alone_v.view(np.int32))
+    for k, ((a, Q), (oi, ov)) in enumerate(zip(reqs, outs)):
+        ev, ei = O.rmq_brute(arrays[a][0], Q)
+        np.testing.assert_array_equal(oi, ei, err_msg=f"request {k}")
+        np.testing.assert_array_equal(ov.view(np.int32), ev.view(np.int32), err_msg=f"request {k}")
+    assert st["requests_done"] == 49 and st["failed"] == 0
+    assert st["launches"] <= 49 and st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert min(lats) > 0
+
+
+def test_memory_backpressure_and_oversized_request(arrays, gpu):
+    budget = 1 << 20  # 1 MiB of request buffers in flight
+    with R.Server(slots=2, mem_budget_bytes=budget) as sv:
+        aid = sv.add_array(arrays[2][1], 256)
+        n = arrays[2][0].size
+        Qs = [O.gen_queries(40_000, n, -7, 90 + k) for k in range(12)]  # ~0.6 MiB each: one at a time
+        outs = [np.empty(40_000, np.int32) for _ in Qs]
+        tks = [sv.submit(aid, Q, out_idx=o) for Q, o in zip(Qs, outs)]
+        big = O.gen_queries(200_000, n, -1, 7)  # 2.4 MiB: can never fit
+        big_out = np.empty(200_000, np.int32)
+        tk_big = sv.submit(aid, big, out_idx=big_out)
+        for tk in tks:
+            assert sv.wait(tk)[0] == R.RMQ_OK
+        assert sv.wait(tk_big)[0] == R.RMQ_ERR_OOM
+        st = sv.stats()
+    assert st["peak_inflight_bytes"] <= budget and st["failed"] == 1 and st["requests_done"] == 12
+    for Q, o in zip(Qs, outs):
+        np.testing.assert_array_equal(o, O.rmq_brute(arrays[2][0], Q)[1])
+
+
+def test_invalid_server_arguments(gpu):
+    lib = R.lib()
+    h = ctypes.c_void_p()
+    cfg = R._native.ServerConfig(-1, 0, 0, 0, 0, 0, R.QueryOpts())
+    assert lib.rmq_server_create(ctypes.byref(cfg), ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    with R.Server() as sv:
+        with pytest.raises(R.RMQError):
+            sv.submit(99, np.zeros((4, 2), np.int32), out_idx=np.empty(4, np.int32))
