@@ -234,6 +234,12 @@ int rmq_gen_array(float *d_A, int64_t n, uint64_t seed, int32_t kind, hipStream_
 int rmq_gen_array_host(float *A, int64_t n, uint64_t seed, int32_t kind);
 int rmq_gen_queries_host(int32_t *Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, int32_t nthreads);
 int rmq_gen_queries(int32_t *d_Q, int64_t q, int32_t n, int32_t lr, uint64_t seed, hipStream_t s);
+/* The reference's own query stream (random_queries_par_cpu, src/rand.cuh:116-201),
+ * call for call: mt19937(seed*tid) per chunk of ceil(q/nt) queries, the
+ * standard library's uniform_int / lognormal distributions, lr > 0 and
+ * -1..-5 with the reference's meanings (lognormal sigma 0.3, scales n^0.6,
+ * n^0.3, n/2^8, n/2^15).  Depends on nt, as the reference's does. */
+int rmq_gen_queries_ref(int32_t *Q, int64_t q, int32_t n, int32_t lr, int32_t seed, int32_t nt);
 
 /* Static description of a return code. */
 const char *rmq_strerror(int code);

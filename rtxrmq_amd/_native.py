@@ -92,6 +92,7 @@ def _declare(lib):
         "rmq_gen_array_host": (ctypes.c_int, [_vp, _i64, _u64, _i32]),
         "rmq_gen_queries_host": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _i32]),
         "rmq_gen_queries": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _vp]),
+        "rmq_gen_queries_ref": (ctypes.c_int, [_vp, _i64, _i32, _i32, _i32, _i32]),
         "rmq_plan_create": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _i32, _vp,
                                            ctypes.POINTER(_vp)]),
         "rmq_plan_launch": (ctypes.c_int, [_vp, _vp]),

@@ -55,6 +55,7 @@ struct Args {
     std::string time_file, power_file;
     int short_max = 0, group_lanes = 0, long_lanes = 0;
     int array_kind = RMQ_GEN_UNIFORM;
+    bool qgen_ref = false;  // the reference's own query stream (src/rand.cuh:116-201)
 };
 
 void usage() {
@@ -81,7 +82,10 @@ void usage() {
             "  --short-max <len>     ranges up to this length use the group scan (alg 5/105)\n"
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
-            "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n",
+            "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
+            "  --qgen <gen>          ours (default: counter-based, thread-count independent) |\n"
+            "                        ref (the reference's mt19937(seed*tid) stream; depends on --nt;\n"
+            "                        lr -1..-5 as the reference defines them)\n",
             kAlgName[0], kAlgName[1], kAlgName[2], kAlgName[5]);
 }
 
@@ -106,7 +110,7 @@ bool parse(int argc, char **argv, Args &a) {
     a.bs = std::min(a.n, 1 << 15);
     a.nb = a.n / a.bs;
     a.seed = (int)time(nullptr);
-    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG };
+    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG, O_QGEN };
     static struct option opts[] = {
         {"bs", required_argument, nullptr, O_BS},         {"nb", required_argument, nullptr, O_NB},
         {"reps", required_argument, nullptr, O_REPS},     {"dev", required_argument, nullptr, O_DEV},
@@ -115,7 +119,7 @@ bool parse(int argc, char **argv, Args &a) {
         {"save-power", optional_argument, nullptr, O_POWER},
         {"short-max", required_argument, nullptr, O_SHORT}, {"group-lanes", required_argument, nullptr, O_GROUP},
         {"array", required_argument, nullptr, O_ARRAY},   {"long-lanes", required_argument, nullptr, O_LONG},
-        {nullptr, 0, nullptr, 0}};
+        {"qgen", required_argument, nullptr, O_QGEN},     {nullptr, 0, nullptr, 0}};
     int opt, idx;
     optind = 5;
     while ((opt = getopt_long(argc, argv, "", opts, &idx)) != -1) {
@@ -132,6 +136,10 @@ bool parse(int argc, char **argv, Args &a) {
         case O_SHORT: a.short_max = atoi(optarg); break;
         case O_GROUP: a.group_lanes = atoi(optarg); break;
         case O_LONG: a.long_lanes = atoi(optarg); break;
+        case O_QGEN:
+            if (!strcmp(optarg, "ref")) a.qgen_ref = true;
+            else if (strcmp(optarg, "ours")) return false;
+            break;
         case O_ARRAY:
             if (!strcmp(optarg, "uniform")) a.array_kind = RMQ_GEN_UNIFORM;
             else if (!strcmp(optarg, "ties16")) a.array_kind = RMQ_GEN_TIES16;
@@ -293,7 +301,11 @@ int main(int argc, char **argv) {
     printf("Generating q=%-10i queries...........", a.q);
     fflush(stdout);
     std::vector<int32_t> hQ(2 * (size_t)a.q);
-    if (int rc = rmq_gen_queries_host(hQ.data(), a.q, a.n, a.lr, (uint64_t)a.seed, a.nt)) die("rmq_gen_queries_host", rc);
+    if (a.qgen_ref) {
+        if (int rc = rmq_gen_queries_ref(hQ.data(), a.q, a.n, a.lr, a.seed, a.nt)) die("rmq_gen_queries_ref", rc);
+    } else if (int rc = rmq_gen_queries_host(hQ.data(), a.q, a.n, a.lr, (uint64_t)a.seed, a.nt)) {
+        die("rmq_gen_queries_host", rc);
+    }
     int32_t *dQ = nullptr;
     if (need_gpu) {
         hip_ok(hipMalloc(&dQ, sizeof(int32_t) * 2 * (size_t)a.q), "hipMalloc(Q)");

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -659,6 +660,70 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     out->from_cache = 0;
     snprintf(out->key, sizeof(out->key), "%s", key);
     cache_store(cache_path, key, *out);
+    return RMQ_OK;
+}
+
+// ===========================================================================
+// the reference's own query stream (random_queries_par_cpu, src/rand.cuh:116-201)
+// ===========================================================================
+// Reproduces the reference's generator call for call, for reruns of its
+// sweeps on the same queries: per "thread" tid in [0, nt) an mt19937 seeded
+// with seed*tid fills the chunk [tid*ceil(q/nt), ...) (src/rand.cuh:119-123);
+// lr > 0 fixed lengths (:116-133), -1 uniform lengths in [1, n] (:136-154),
+// -2..-5 lognormal lengths with sigma 0.3, redrawn until in [1, n], at the
+// scales n^0.6, n^0.3, n/2^8, n/2^15 (:156-177, :188-199).  The distributions
+// are the standard library's (mt19937, uniform_int_distribution,
+// lognormal_distribution), so the stream is the reference's wherever it is
+// built against the same C++ standard library.  The stream depends on nt,
+// as in the reference.
+RMQ_API int rmq_gen_queries_ref(int32_t *Q, int64_t q, int32_t n, int32_t lr, int32_t seed, int32_t nt) {
+    if (!Q || q < 0 || q > INT32_MAX || n <= 0 || lr == 0 || lr > n || lr < -5 || nt <= 0) return RMQ_ERR_INVALID_ARG;
+    const int qi = (int)q;
+    const int chunk = (qi + nt - 1) / nt;
+    auto fill = [&](int tid) {
+        std::mt19937 gen((uint32_t)seed * (uint32_t)tid);  // the reference's int product, wrapped
+        const int begin = chunk * tid;
+        const int end = begin + chunk;
+        if (lr > 0) {
+            for (int i = begin; i < qi && i < end; ++i) {
+                std::uniform_int_distribution<int> lrand(0, n - 1 - (lr - 1));
+                const int l = lrand(gen);
+                Q[2 * (int64_t)i] = l;
+                Q[2 * (int64_t)i + 1] = l + (lr - 1);
+            }
+        } else if (lr == -1) {
+            std::uniform_int_distribution<int> dist(1, n);
+            for (int i = begin; i < qi && i < end; ++i) {
+                const int len = dist(gen);
+                std::uniform_int_distribution<int> lrand(0, n - 1 - (len - 1));
+                const int l = lrand(gen);
+                Q[2 * (int64_t)i] = l;
+                Q[2 * (int64_t)i + 1] = l + (len - 1);
+            }
+        } else {
+            int scale = 1;
+            if (lr == -2) scale = (int)std::pow((double)n, 0.6);
+            if (lr == -3) scale = (int)std::pow((double)n, 0.3);
+            if (lr == -4) scale = std::max(1, n / (1 << 8));
+            if (lr == -5) scale = std::max(1, n / (1 << 15));
+            std::lognormal_distribution<double> dist(std::log((double)scale), 0.3);
+            for (int i = begin; i < qi && i < end; ++i) {
+                int len;
+                do {
+                    const double x = dist(gen);
+                    len = (x >= 2147483648.0 || x != x) ? INT_MIN : (int)x;  // x86 conversion of out-of-range values
+                } while (len <= 0 || len > n);
+                std::uniform_int_distribution<int> lrand(0, n - 1 - (len - 1));
+                const int l = lrand(gen);
+                Q[2 * (int64_t)i] = l;
+                Q[2 * (int64_t)i + 1] = l + (len - 1);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(fill, t);
+    fill(0);
+    for (auto &th : pool) th.join();
     return RMQ_OK;
 }
 

@@ -205,6 +205,13 @@ def gen_array(n: int, seed: int, kind: str = "uniform", device="cuda", stream=No
     return A
 
 
+def gen_queries_ref(q: int, n: int, lr: int, seed: int, nt: int) -> np.ndarray:
+    """The reference's query stream (random_queries_par_cpu, src/rand.cuh:116-201); depends on nt."""
+    Q = np.empty((q, 2), np.int32)
+    N.check(N.lib().rmq_gen_queries_ref(Q.ctypes.data, q, n, lr, seed, nt), "rmq_gen_queries_ref")
+    return Q
+
+
 def gen_queries_device(q: int, n: int, lr: int, seed: int, device="cuda", stream=None):
     """Device queries from the same counter-based stream (rmq_gen_queries); (q, 2) int32 tensor."""
     torch = _torch()

@@ -1,0 +1,82 @@
+"""The ./rtxrmq command line: same arguments, alg ids, options and CSV rows as
+the reference driver (src/main.cu:51-171, src/tools.h:28-189,268-336)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rtxrmq_amd import _native as N
+
+CLI = os.path.join(N.PKG, "bin", "rtxrmq")
+
+
+def run(*args, cwd=None):
+    return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built")
+
+
+def test_usage_and_argument_errors():
+    r = run()
+    assert r.returncode == 1 and "usage: ./rtxrmq <n> <q> <lr> <alg>" in r.stderr
+    assert run(1000, 10, 0, 5).returncode == 1          # lr == 0 rejected (src/tools.h:87-90)
+    r = run(1000, 10, 2000, 5)                          # lr > n rejected (src/tools.h:91-94)
+    assert r.returncode == 1 and "lr=2000 > n=1000" in r.stderr
+    r = run(1000, 10, -1, 3)                            # RT-core algs are outside this engine
+    assert r.returncode == 2 and "outside this engine" in r.stderr
+    r = run(1000, 10, -1, 1)                            # HRMQ is the baseline harness's
+    assert r.returncode == 2 and "HRMQ" in r.stderr
+    assert run(1000, 10, -1, 5, "--array", "nope").returncode == 1
+
+
+@pytest.mark.parametrize("alg", [0, 100])
+def test_cpu_base_writes_reference_csv_row(tmp_path, alg):
+    csv = tmp_path / "t.csv"
+    r = run(3000, 2000, -3, alg, "--nt", 2, "--reps", 2, "--seed", 9, f"--save-time={csv}")
+    assert r.returncode == 0, r.stderr
+    assert "Benchmark Finished" in r.stdout
+    row = csv.read_text().strip().split(",")
+    # device,alg,reps,n,bs,q,lr,t,q/s,ns/q,construction,outbuffer,tempbuffer
+    assert len(row) == 13 and row[0].startswith("CPU ") and row[1] == "[CPU] BASE"
+    assert row[2:7] == ["2", "3000", "0", "2000", "-3"]  # bs zeroed for non-block algs (src/tools.h:163-173)
+    assert float(row[8]) > 0 and float(row[9]) > 0
+
+
+def test_reference_query_stream_option(tmp_path):
+    csv = tmp_path / "r.csv"
+    r = run(5000, 3000, -2, 100, "--qgen", "ref", "--nt", 4, "--reps", 1, "--seed", 11, f"--save-time={csv}")
+    assert r.returncode == 0, r.stderr
+    assert run(5000, 10, -1, 0, "--qgen", "bogus").returncode == 1
+
+
+def test_nb_overrides_bs_and_bs_clamps_to_n(tmp_path):
+    # only argument handling runs here (CPU alg); bs/nb semantics are printed
+    r = run(1000, 10, 5, 0, "--nb", 7, "--nt", 1, "--reps", 1, "--seed", 1)
+    assert r.returncode == 0 and "bs   = 0" in r.stdout  # CPU alg: zeroed like the reference
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", [5, 105, 2, 102])
+@pytest.mark.parametrize("lr", [-1, -3, 77])
+def test_gpu_algs_pass_check(alg, lr, tmp_path, gpu):
+    csv = tmp_path / "g.csv"
+    r = run(1 << 20, 1 << 16, lr, alg, "--bs", 1000, "--reps", 3, "--seed", 1321, "--nt", 8, "--check",
+            f"--save-time={csv}")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "pass (0 mismatches)" in r.stdout
+    row = csv.read_text().strip().split(",")
+    assert len(row) == 13 and row[1] == ("[GPU] RTX_blocks" if alg % 100 == 5 else "[GPU] BASE")
+    if alg % 100 == 5:
+        assert row[4] == "1000" and float(row[11]) > 0  # bs and the index size (MB)
+
+
+@pytest.mark.gpu
+def test_gpu_tie_arrays_and_nb(gpu):
+    for kind in ("ties16", "desc", "const"):
+        r = run(300_007, 50_000, -2, 105, "--nb", 97, "--reps", 1, "--seed", 4, "--array", kind, "--check")
+        assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
