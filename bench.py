@@ -47,16 +47,28 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(Q: np.ndarray, bs: int, short_max: int, out_bytes: int = 8) -> tuple[float, float, int, int]:
-    """(bytes in the long kernel, bytes in the short kernel, n_long, n_short) per pass."""
+def algorithmic_bytes(Q: np.ndarray, bs: int, short_max: int, out_bytes: int = 8,
+                      edges: bool = False) -> tuple[float, float, int, int]:
+    """(bytes in the long kernel, bytes in the short kernel, n_long, n_short) per pass.
+
+    Scan path: B = 8 + 4*S + T + out.  With edge tables a long range (more than
+    max(short_max, 32) elements) scans nothing: B = 8 + 16 (suffix and prefix
+    pairs) + T + out, T = 16 when whole blocks (or, inside one block, whole
+    32-element sub-blocks) lie between the two edges."""
     l = Q[:, 0].astype(np.int64)
     r = Q[:, 1].astype(np.int64)
     ln = r - l + 1
     lB, rB = l // bs, r // bs
-    same = lB == rB
-    S = np.where(same, ln, (bs - l % bs) + (r % bs + 1))
-    T = np.where(rB - lB > 1, 16, 0)
-    B = 8 + 4 * S + T + out_bytes
+    if edges:
+        short_max = max(short_max, 32)
+        jl, jr = (l - lB * bs) // 32, (r - lB * bs) // 32
+        mid = np.where(lB != rB, rB - lB > 1, jr - jl > 1)
+        B = 8 + 16 + np.where(mid, 16, 0) + out_bytes
+    else:
+        same = lB == rB
+        S = np.where(same, ln, (bs - l % bs) + (r % bs + 1))
+        T = np.where(rB - lB > 1, 16, 0)
+        B = 8 + 4 * S + T + out_bytes
     is_long = ln > short_max
     short_B = 8 + 4 * ln + out_bytes
     return float(B[is_long].sum()), float(short_B[~is_long].sum()), int(is_long.sum()), int((~is_long).sum())
@@ -145,12 +157,13 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
     return steps_out, time.perf_counter() - t_start
 
 
-def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for) -> dict:
+def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -> dict:
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel."""
     per_dist = {}
     kernels = {"k_query_long": [0.0, 0.0], "k_query_short": [0.0, 0.0]}  # bytes, ms
     for lr in DISTS:
-        lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs_for[lr], opts_for[lr].short_max)
+        lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs_for[lr], opts_for[lr].short_max,
+                                           edges=bool(edges_for and edges_for[lr]))
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]
         sms = [s[lr]["short_ms"] for s in steps_out]
@@ -214,11 +227,11 @@ def run_batched(R, torch, dev, args) -> dict:
     torch.cuda.synchronize()
     big = int(np.argmax(ns))
     pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096])
-    opts, bs = pick["opts"], pick["bs"]
-    log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} short_max={opts.short_max} "
-        f"group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
+    opts, bs, flags = pick["opts"], pick["bs"], pick["flags"]
+    log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} edges={bool(flags)} "
+        f"short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
     t0 = time.perf_counter()
-    indexes = [R.RMQIndex(A, bs) for A in A_views]
+    indexes = [R.RMQIndex(A, bs, flags=flags) for A in A_views]
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     outs_p = [out_plan[q_off[i]:q_off[i + 1]] for i in range(m)]
@@ -255,7 +268,7 @@ def run_batched(R, torch, dev, args) -> dict:
     plan.close()
     for ix in indexes:
         ix.close()
-    return {"arrays": m, "elements": int(ns.sum()), "queries": total_q, "bs": bs,
+    return {"arrays": m, "elements": int(ns.sum()), "queries": total_q, "bs": bs, "edge_tables": bool(flags),
             "opts": {"short_max": opts.short_max, "group_lanes": opts.group_lanes, "long_lanes": opts.long_lanes},
             "work_items": info["items"], "rmq_per_s_graph": total_q / t_graph, "rmq_per_s_plan": total_q / t_eager,
             "rmq_per_s_loop": total_q / t_loop, "speedup_vs_loop": t_loop / t_graph,
@@ -281,8 +294,9 @@ def run_served(R, torch, dev, args) -> dict:
     sample = torch.cat([R.gen_queries_device(1 << 20, int(ns[big]), lr, 77, device=dev) for lr in (-1, -7, -8)])
     pick = R.tune(arrays[big], sample, bs_list=[16, 32, 64, 128, 256, 1024], cache_path=args.tune_cache or None,
                   refresh=args.retune)
-    bs, opts = pick["bs"], pick["opts"]
-    log(f"served: bs={bs} short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
+    bs, opts, flags = pick["bs"], pick["opts"], pick["flags"]
+    log(f"served: bs={bs} edges={bool(flags)} short_max={opts.short_max} group_lanes={opts.group_lanes} "
+        f"long_lanes={opts.long_lanes}")
     # payload pool: per array, serve_payloads query batches in pinned host memory
     payloads = []
     for i, n in enumerate(ns):
@@ -300,7 +314,7 @@ def run_served(R, torch, dev, args) -> dict:
     max_q = int(pq.max())
 
     # plain per-request loop on the same handles
-    indexes = [R.RMQIndex(A, bs) for A in arrays]
+    indexes = [R.RMQIndex(A, bs, flags=flags) for A in arrays]
     plain_out = {}
     out_pin = torch.empty(max_q, dtype=torch.int32, pin_memory=True)
     order = rng.integers(0, len(payloads), args.serve_plain_requests)
@@ -323,7 +337,7 @@ def run_served(R, torch, dev, args) -> dict:
     log(f"served: plain per-request loop {plain_rate:.4e} RMQ/s over {len(order)} requests")
 
     sv = R.Server(slots=args.serve_slots, opts=opts)
-    ids = [sv.add_array(A, bs) for A in arrays]
+    ids = [sv.add_array(A, bs, flags) for A in arrays]
     ring = [torch.empty(max_q, dtype=torch.int32, pin_memory=True) for _ in range(args.serve_ring)]
 
     def run_stream(n_req, rate_per_s, tag):
@@ -391,7 +405,7 @@ def run_served(R, torch, dev, args) -> dict:
     offered = args.serve_load * req_rate
     poi, poi_rate, poi_bad = run_stream(args.serve_requests, offered, f"poisson {offered:.0f} req/s")
     sv.close()
-    return {"arrays": m, "elements": int(ns.sum()), "bs": bs, "slots": args.serve_slots,
+    return {"arrays": m, "elements": int(ns.sum()), "bs": bs, "edge_tables": bool(flags), "slots": args.serve_slots,
             "request_q": "log-uniform [2^8, 2^22]", "mean_q": float(pq.mean()), "payloads": len(payloads),
             "plain_rmq_per_s": plain_rate, "plain_requests": len(order),
             "saturation": {"rmq_per_s": sat_rate, "requests": sat["requests_done"], "p50_ms": sat["p50_ms"],
@@ -494,24 +508,25 @@ def main() -> int:
         t0 = time.perf_counter()
         picks[lr] = R.tune(A, sample, cache_path=args.tune_cache or None, refresh=args.retune)
         p = picks[lr]
-        log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} short_max={p['opts'].short_max} "
+        log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} edges={bool(p['flags'])} short_max={p['opts'].short_max} "
             f"group_lanes={p['opts'].group_lanes} long_lanes={p['opts'].long_lanes} "
             f"({'cached' if p['from_cache'] else str(p['n_tried']) + ' configs'}, {time.perf_counter() - t0:.1f} s, "
             f"key {p['key']})")
     indexes, build_ms = {}, {}
     for lr in DISTS:
-        bs = picks[lr]["bs"]
-        if bs not in indexes:
+        key = (picks[lr]["bs"], picks[lr]["flags"])
+        if key not in indexes:
             t0 = time.perf_counter()
-            indexes[bs] = R.RMQIndex(A, bs)
+            indexes[key] = R.RMQIndex(A, key[0], flags=key[1])
             torch.cuda.synchronize()
-            build_ms[bs] = (time.perf_counter() - t0) * 1e3
-    index_for = {lr: indexes[picks[lr]["bs"]] for lr in DISTS}
+            build_ms[f"bs={key[0]}{' edges' if key[1] else ''}"] = (time.perf_counter() - t0) * 1e3
+    index_for = {lr: indexes[(picks[lr]["bs"], picks[lr]["flags"])] for lr in DISTS}
     opts_for = {lr: picks[lr]["opts"] for lr in DISTS}
     bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
     steps_out, elapsed = timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, args.steps, args.warmup,
                                    "tuned")
-    tuned = summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for)
+    tuned = summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for,
+                      {lr: picks[lr]["flags"] for lr in DISTS})
     log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
     for ix in indexes.values():
@@ -544,6 +559,7 @@ def main() -> int:
                                "per distribution",
                    "n": n, "q": q, "dists": [DIST_NAMES[lr] for lr in DISTS],
                    "bs": {DIST_NAMES[lr]: bs_for[lr] for lr in DISTS},
+                   "edge_tables": {DIST_NAMES[lr]: bool(picks[lr]["flags"]) for lr in DISTS},
                    "construction_ms": {str(k): v for k, v in build_ms.items()},
                    "tune_sample": min(q, args.tune_sample),
                    "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},

@@ -47,6 +47,9 @@ typedef struct rmq_handle rmq_handle_t;
 /* Array kinds for the generators (the reference draws U(0,1] with cuRAND,
  * src/rand.cuh:15-21; these draw exact grid values from a counter-based
  * stream so that host and device produce the same array). */
+/* rmq_build flags */
+#define RMQ_BUILD_EDGES 1u /* precompute the edge-block searches (32 B per element): see rmq_build */
+
 #define RMQ_GEN_UNIFORM 0 /* k * 2^-24, k uniform in [0, 2^24): U[0,1) exactly      */
 #define RMQ_GEN_TIES16 1  /* integers 0..15 as float (heavy ties)                  */
 #define RMQ_GEN_DESC 2    /* bits 0x3F800000 - 1 - i: strictly descending, positive */
@@ -73,7 +76,12 @@ typedef struct rmq_query_times {
  *      (src/rtx.h:3-22).  Builds the per-block leftmost min/argmin over blocks
  *      of bs elements (nb = ceil(n/bs), src/rtx.h:42) and a sparse table over
  *      them.  d_A must stay alive and unchanged while the handle is used: the
- *      edge-block scans read it.  n <= 2^31-1, 1 <= bs.  flags is reserved (0).
+ *      edge-block scans read it.  n <= 2^31-1, 1 <= bs, d_A 16-byte aligned.
+ *      flags: 0, or RMQ_BUILD_EDGES to also precompute, per element, the
+ *      leftmost (min, argmin) from its block start and to its block end, and
+ *      the same inside 32-element sub-blocks with a sparse table over them
+ *      (32 bytes per element): a range of more than 32 elements is then
+ *      answered by one lane with four O(1) lookups instead of edge scans.
  *      Stream-ordered; the handle is usable on stream s at once, on other
  *      streams after s is synchronised. ---- */
 int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out);
@@ -114,6 +122,7 @@ int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts);
  * distribution).  key is the distribution signature the cache is keyed by. */
 typedef struct rmq_tune_result {
     int32_t bs;
+    uint32_t build_flags; /* 0 or RMQ_BUILD_EDGES */
     rmq_query_opts_t opts;
     float best_ms;      /* one query pass over the tuning sample */
     int32_t n_tried;    /* configurations timed (0 when taken from the cache) */
@@ -124,8 +133,10 @@ typedef struct rmq_tune_result {
 /* ---- autotuner (no counterpart in the reference, whose sweeps are shell
  *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
  *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^3 ..
- *      2^16), then the short-range threshold and lanes per query at the best
- *      block size, and returns the fastest configuration.  Results are cached
+ *      2^16), with and without edge tables (RMQ_BUILD_EDGES), then the
+ *      short-range threshold and lanes per query at the best block size, and
+ *      returns the fastest configuration (build it with rmq_build(..., bs,
+ *      build_flags, ...)).  Results are cached
  *      per (n, length-distribution signature) in the text file cache_path
  *      (NULL: no cache).  d_val / d_idx are scratch outputs of q entries (one
  *      may be NULL).  Synchronising; builds and frees its own handles. ---- */
@@ -201,6 +212,9 @@ typedef struct rmq_server_stats {
 } rmq_server_stats_t;
 int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out);
 int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, int32_t *array_id);
+/* rmq_server_add_array with rmq_build flags (RMQ_BUILD_EDGES). */
+int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, uint32_t flags,
+                            int32_t *array_id);
 int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, int64_t q, int32_t *h_idx,
                       float *h_val, uint64_t *ticket);
 /* Blocks until the request is answered; returns its status (RMQ_ERR_OOM if it

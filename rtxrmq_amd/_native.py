@@ -23,6 +23,7 @@ RMQ_ERR_DEVICE = -4
 RMQ_ERR_BAD_QUERY = -5
 
 GEN_KINDS = {"uniform": 0, "ties16": 1, "desc": 2, "const": 3}
+BUILD_EDGES = 1  # RMQ_BUILD_EDGES
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -50,7 +51,7 @@ class QueryTimes(ctypes.Structure):
 
 class TuneResult(ctypes.Structure):
     """rmq_tune_result_t"""
-    _fields_ = [("bs", _i32), ("opts", QueryOpts), ("best_ms", ctypes.c_float), ("n_tried", _i32),
+    _fields_ = [("bs", _i32), ("build_flags", _u32), ("opts", QueryOpts), ("best_ms", ctypes.c_float), ("n_tried", _i32),
                 ("from_cache", _i32), ("key", ctypes.c_char * 96)]
 
 
@@ -102,6 +103,7 @@ def _declare(lib):
         "rmq_query_loop": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _vp]),
         "rmq_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
         "rmq_server_add_array": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.POINTER(_i32)]),
+        "rmq_server_add_array_ex": (ctypes.c_int, [_vp, _vp, _i64, _i32, _u32, ctypes.POINTER(_i32)]),
         "rmq_server_submit": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp, ctypes.POINTER(_u64)]),
         "rmq_server_wait": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
         "rmq_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats), _i32]),

@@ -28,7 +28,13 @@ struct ArrayDesc {
     const Pair *st;
     uint64_t magic;  // block_magic(bs)
     int32_t n, bs, nb, pad;
+    EdgeTables edges;  // pre_b == nullptr: no edge tables
 };
+
+inline ArrayDesc desc_of(const rmq_handle *h) {
+    ArrayDesc d{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0, h->edges};
+    return d;
+}
 
 struct WorkItem {
     const int2 *Q;
@@ -58,13 +64,22 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
         if (it >= (unsigned)n_items) break;
         const WorkItem w = items[it];
         const ArrayDesc d = descs[w.array];
+        const bool edges = d.edges.pre_b != nullptr;  // uniform across the wave
+        const int sm = edges ? max(short_max, kSub) : short_max;
         const int64_t tiles = ((int64_t)w.count + kWave - 1) / kWave;
         for (int64_t t = 0; t < tiles; ++t) {
-            const TileMasks m = load_tile(w.Q, w.count, d.n, short_max, t, lane, true, w.out_v, w.out_i);
+            const TileMasks m = load_tile(w.Q, w.count, d.n, sm, t, lane, true, w.out_v, w.out_i);
             answer_class<GS, false>(d.A, d.bs, d.magic, d.nb, d.st, m.shorts, m, lane, t * kWave, w.out_v,
                                     w.out_i, stage[wv]);
-            answer_class<GL, true>(d.A, d.bs, d.magic, d.nb, d.st, m.longs, m, lane, t * kWave, w.out_v,
-                                   w.out_i, stage[wv]);
+            if (edges) {
+                if ((m.longs >> lane) & 1) {
+                    const Pair b = answer_edges(d.edges, d.st, d.nb, d.bs, d.magic, m.l, m.r);
+                    write_result(w.out_v, w.out_i, t * kWave + lane, b.v, b.i);
+                }
+            } else {
+                answer_class<GL, true>(d.A, d.bs, d.magic, d.nb, d.st, m.longs, m, lane, t * kWave, w.out_v,
+                                       w.out_i, stage[wv]);
+            }
         }
     }
 }
@@ -183,7 +198,7 @@ RMQ_API int rmq_plan_create(const rmq_handle_t *const *handles, int32_t m, const
             plan_release(p);
             return h && h->device != p->device ? RMQ_ERR_DEVICE : RMQ_ERR_INVALID_ARG;
         }
-        descs[i] = ArrayDesc{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0};
+        descs[i] = desc_of(h);
         for (int64_t off = 0; off < qs[i]; off += item_queries) {
             const int32_t c = (int32_t)std::min<int64_t>(item_queries, qs[i] - off);
             items.push_back(WorkItem{reinterpret_cast<const int2 *>(d_Qs[i]) + off,
@@ -636,8 +651,10 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
     return RMQ_OK;
 }
 
-RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, int32_t *array_id) {
-    if (!sv || !array_id || !d_A || n <= 0 || n > INT32_MAX || bs <= 0) return RMQ_ERR_INVALID_ARG;
+RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, uint32_t flags,
+                                    int32_t *array_id) {
+    if (!sv || !array_id || !d_A || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES))
+        return RMQ_ERR_INVALID_ARG;
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
     if (dev != sv->device) return RMQ_ERR_DEVICE;
@@ -648,7 +665,8 @@ RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, 
     const int64_t nb = (n + bs - 1) / bs;
     int64_t levels = 0;
     while (((int64_t)1 << levels) <= nb) ++levels;
-    const int64_t table = (int64_t)sizeof(Pair) * nb * levels;
+    int64_t table = (int64_t)sizeof(Pair) * nb * levels;
+    if (flags & RMQ_BUILD_EDGES) table += (int64_t)sizeof(Pair) * (4 * n + n / 2);  // edge tables, generous
     size_t free_b = 0, total_b = 0;
     int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
     if (rc != RMQ_OK) return rc;
@@ -656,13 +674,13 @@ RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, 
     if (table + (sv->budget - sv->inflight) > (int64_t)free_b) return RMQ_ERR_OOM;
     rmq_handle_t *h = nullptr;
     hipStream_t s = sv->slots[0].stream;
-    rc = rmq_build(d_A, n, bs, 0, s, &h);
+    rc = rmq_build(d_A, n, bs, flags, s, &h);
     if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
     if (rc != RMQ_OK) {
         rmq_free(h);
         return rc;
     }
-    const ArrayDesc d{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0};
+    const ArrayDesc d = desc_of(h);
     const int32_t id = (int32_t)sv->arrays.size();
     // a new entry is written beyond every entry an in-flight launch can read
     rc = hip_rc(hipMemcpy(sv->d_descs + id, &d, sizeof(d), hipMemcpyHostToDevice));
@@ -673,6 +691,10 @@ RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, 
     sv->arrays.push_back(h);
     *array_id = id;
     return RMQ_OK;
+}
+
+RMQ_API int rmq_server_add_array(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, int32_t *array_id) {
+    return rmq_server_add_array_ex(sv, d_A, n, bs, 0, array_id);
 }
 
 RMQ_API int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, int64_t q, int32_t *h_idx,

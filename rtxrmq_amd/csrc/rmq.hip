@@ -76,6 +76,100 @@ __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__
     cur[i] = (y.v < x.v) ? y : x;
 }
 
+// Edge tables, step 1: one thread per sub-block (32 elements, aligned to its
+// block's start): leftmost prefix and suffix (min, argmin) inside it and its
+// minimum.  Prefix: strict '<' going right; suffix: '<=' going left, so both
+// keep the leftmost of equal values.  Sub-blocks past a partial last block
+// are empty and get the sentinel (they are never read by a valid query).
+__global__ void k_sub_edges(const float *__restrict__ A, int n, int bs, int nsub, int64_t subs_total,
+                            Pair *__restrict__ pre_s, Pair *__restrict__ suf_s, Pair *__restrict__ submin) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= subs_total) return;
+    const int64_t b = g / nsub, j = g - b * nsub;
+    const int64_t start = b * bs + j * kSub;
+    const int64_t end = min(min(start + kSub, (b + 1) * bs), (int64_t)n);
+    if (start >= end) {
+        submin[g] = Pair{INFINITY, INT_MAX};
+        return;
+    }
+    Pair m{A[start], (int32_t)start};
+    for (int64_t i = start; i < end; ++i) {
+        const float x = A[i];
+        if (x < m.v) m = Pair{x, (int32_t)i};
+        if (pre_s) pre_s[i] = m;
+    }
+    submin[g] = m;
+    if (!suf_s) return;
+    m = Pair{A[end - 1], (int32_t)(end - 1)};
+    for (int64_t i = end - 1; i >= start; --i) {
+        const float x = A[i];
+        if (!(m.v < x)) m = Pair{x, (int32_t)i};
+        suf_s[i] = m;
+    }
+}
+
+// Step 2: one thread per block: exclusive prefix and suffix minima over the
+// block's sub-block minima (index tie-break, so the leftmost wins).
+__global__ void k_block_sub_scan(const Pair *__restrict__ submin, int nb, int nsub, Pair *__restrict__ excl_pre,
+                                 Pair *__restrict__ excl_suf) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t base = (int64_t)b * nsub;
+    float rv = INFINITY;
+    int ri = INT_MAX;
+    for (int j = 0; j < nsub; ++j) {
+        excl_pre[base + j] = Pair{rv, ri};
+        const Pair x = submin[base + j];
+        take_if_better(x.v, x.i, rv, ri);
+    }
+    rv = INFINITY;
+    ri = INT_MAX;
+    for (int j = nsub - 1; j >= 0; --j) {
+        excl_suf[base + j] = Pair{rv, ri};
+        const Pair x = submin[base + j];
+        take_if_better(x.v, x.i, rv, ri);
+    }
+}
+
+// Step 3: one thread per element: block-level prefix = (sub-blocks before,
+// prefix inside its sub-block); block-level suffix = (suffix inside its
+// sub-block, sub-blocks after).  The earlier part wins ties.
+__global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pair *__restrict__ pre_s,
+                              const Pair *__restrict__ suf_s, const Pair *__restrict__ excl_pre,
+                              const Pair *__restrict__ excl_suf, Pair *__restrict__ pre_b, Pair *__restrict__ suf_b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = block_of(i, magic);
+    const int64_t g = (int64_t)b * nsub + ((i - b * bs) >> kSubShift);
+    Pair p = excl_pre[g];
+    const Pair x = pre_s[i];
+    take_if_better(x.v, x.i, p.v, p.i);
+    pre_b[i] = p;
+    Pair sf = suf_s[i];
+    const Pair e = excl_suf[g];
+    take_if_better(e.v, e.i, sf.v, sf.i);
+    suf_b[i] = sf;
+}
+
+// Ranges of more than short_max (>= 32) elements with edge tables: one lane
+// per query, four O(1) lookups (answer_edges), no scan and no reduction.
+__global__ __launch_bounds__(kWG) void k_query_edges(int n, int bs, uint64_t magic, int nb,
+                                                     const Pair *__restrict__ st, EdgeTables e,
+                                                     const int2 *__restrict__ Q, int64_t q, int short_max,
+                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
+    for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
+        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, false, out_v, out_i);
+        if ((m.longs >> lane) & 1) {
+            const Pair best = answer_edges(e, st, nb, bs, magic, m.l, m.r);
+            write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
+        }
+    }
+}
+
 // Tile kernel: each wave takes tiles of 64 consecutive queries and answers
 // the ones of its class (LONG = more than short_max elements) with
 // answer_class.  The short-class kernel also writes (NaN, -1) for invalid
@@ -204,15 +298,28 @@ int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int g
     return hip_rc(hipGetLastError());
 }
 
+int effective_short_max(const rmq_handle *h, const rmq_query_opts_t &o);
+
 int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                  hipStream_t s) {
-    return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), o.short_max, val, idx, s);
+    return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), effective_short_max(h, o), val, idx, s);
+}
+
+// With edge tables every range of more than 32 elements is O(1), so the
+// short class is widened to at least 32 elements.
+int effective_short_max(const rmq_handle *h, const rmq_query_opts_t &o) {
+    return h->edges.pre_b ? std::max(o.short_max, kSub) : o.short_max;
 }
 
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                 hipStream_t s) {
     int grid = tiles_grid(q, 8);
     if (o.long_wgs > 0) grid = (int)std::min<int64_t>(o.long_wgs, grid);
+    if (h->edges.pre_b) {
+        k_query_edges<<<grid, kWG, 0, s>>>((int)h->n, h->bs, block_magic(h->bs), (int)h->nb, h->st, h->edges, Q, q,
+                                           effective_short_max(h, o), val, idx);
+        return hip_rc(hipGetLastError());
+    }
     return launch_tiles<true>(h, Q, q, o.long_lanes, grid, o.short_max, val, idx, s);
 }
 
@@ -263,10 +370,61 @@ using namespace rmqx;
 // C ABI
 // ===========================================================================
 
+namespace {
+
+// Edge tables in one allocation: pre_b, suf_b (8 B per element each), and when
+// bs > 32 also pre_s, suf_s (the same inside sub-blocks), the sub-block sparse
+// table and the two per-sub-block exclusive scans.
+int build_edges(rmq_handle *h, hipStream_t s) {
+    const int64_t n = h->n;
+    const int bs = h->bs;
+    const int nsub = (bs + kSub - 1) / kSub;
+    const int64_t subs = h->nb * nsub;
+    int levels_s = 0;
+    while ((1 << levels_s) <= nsub) ++levels_s;  // floor(log2(nsub)) + 1
+    const bool two_level = bs > kSub;
+    const int64_t per_elem = two_level ? 4 : 2;
+    const int64_t bytes = (int64_t)sizeof(Pair) * (per_elem * n + (int64_t)levels_s * subs + 2 * subs);
+    int rc = hip_rc(hipMalloc(&h->edge_mem, (size_t)bytes));
+    if (rc != RMQ_OK) return rc;
+    h->edge_bytes = bytes;
+    Pair *p = static_cast<Pair *>(h->edge_mem);
+    Pair *pre_b = p, *suf_b = p + n;
+    Pair *pre_s = two_level ? p + 2 * n : pre_b;
+    Pair *suf_s = two_level ? p + 3 * n : suf_b;
+    Pair *st_s = p + per_elem * n;
+    Pair *excl_pre = st_s + (int64_t)levels_s * subs;
+    Pair *excl_suf = excl_pre + subs;
+    k_sub_edges<<<(unsigned)((subs + 255) / 256), 256, 0, s>>>(h->A, (int)n, bs, nsub, subs, pre_s, suf_s, st_s);
+    rc = hip_rc(hipGetLastError());
+    for (int k = 1; rc == RMQ_OK && k < levels_s; ++k) {
+        const int64_t count = subs - ((int64_t)1 << k) + 1;
+        k_sparse_level<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(st_s + (int64_t)(k - 1) * subs,
+                                                                      st_s + (int64_t)k * subs, (int)count,
+                                                                      1 << (k - 1));
+        rc = hip_rc(hipGetLastError());
+    }
+    if (rc == RMQ_OK && two_level) {
+        k_block_sub_scan<<<(unsigned)((h->nb + 255) / 256), 256, 0, s>>>(st_s, (int)h->nb, nsub, excl_pre, excl_suf);
+        rc = hip_rc(hipGetLastError());
+        if (rc == RMQ_OK) {
+            k_block_edges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, bs, block_magic(bs), nsub, pre_s, suf_s,
+                                                                      excl_pre, excl_suf, pre_b, suf_b);
+            rc = hip_rc(hipGetLastError());
+        }
+    }
+    h->edges = EdgeTables{pre_b, suf_b, pre_s, suf_s, st_s, subs, nsub, levels_s};
+    return rc;
+}
+
+}  // namespace
+
 RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out) {
     if (!out) return RMQ_ERR_INVALID_ARG;
     *out = nullptr;
-    if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || flags != 0) return RMQ_ERR_INVALID_ARG;
+    if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES) ||
+        (reinterpret_cast<uintptr_t>(d_A) & 15))
+        return RMQ_ERR_INVALID_ARG;
     const int64_t nb = (n + bs - 1) / bs;
     int32_t levels = 0;
     while (((int64_t)1 << levels) <= nb) ++levels;  // floor(log2(nb)) + 1
@@ -293,8 +451,11 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
                                                            count, half);
         rc = hip_rc(hipGetLastError());
     }
+    h->flags = flags;
+    if (rc == RMQ_OK && (flags & RMQ_BUILD_EDGES)) rc = build_edges(h, s);
     if (rc != RMQ_OK) {
         if (h->st) (void)hipFree(h->st);
+        if (h->edge_mem) (void)hipFree(h->edge_mem);
         delete h;
         return rc;
     }
@@ -341,6 +502,7 @@ RMQ_API int rmq_query_check(const int32_t *d_Q, int64_t q, int64_t n, hipStream_
 RMQ_API void rmq_free(rmq_handle_t *h) {
     if (!h) return;
     if (h->st) (void)hipFree(h->st);
+    if (h->edge_mem) (void)hipFree(h->edge_mem);
     delete h;
 }
 
@@ -351,7 +513,7 @@ RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int6
     if (bs) *bs = h->bs;
     if (nb) *nb = h->nb;
     if (levels) *levels = h->levels;
-    if (bytes) *bytes = h->st_bytes;
+    if (bytes) *bytes = h->st_bytes + h->edge_bytes;
     return RMQ_OK;
 }
 
@@ -541,9 +703,11 @@ bool cache_lookup(const char *path, const char *key, rmq_tune_result_t *out) {
     bool found = false;
     while (fgets(line, sizeof(line), f)) {
         rmq_tune_result_t r{};
-        if (sscanf(line, "%255s %d %d %d %d %f", k, &r.bs, &r.opts.short_max, &r.opts.group_lanes,
-                   &r.opts.long_lanes, &r.best_ms) == 6 &&
-            strcmp(k, key) == 0) {
+        unsigned flags = 0;
+        const int got = sscanf(line, "%255s %d %d %d %d %f %u", k, &r.bs, &r.opts.short_max, &r.opts.group_lanes,
+                               &r.opts.long_lanes, &r.best_ms, &flags);
+        if (got >= 6 && strcmp(k, key) == 0) {
+            r.build_flags = got == 7 ? flags : 0u;
             snprintf(r.key, sizeof(r.key), "%s", key);
             *out = r;  // last entry for a key wins
             found = true;
@@ -556,7 +720,8 @@ bool cache_lookup(const char *path, const char *key, rmq_tune_result_t *out) {
 void cache_store(const char *path, const char *key, const rmq_tune_result_t &r) {
     FILE *f = path ? fopen(path, "a") : nullptr;
     if (!f) return;
-    fprintf(f, "%s %d %d %d %d %.6f\n", key, r.bs, r.opts.short_max, r.opts.group_lanes, r.opts.long_lanes, r.best_ms);
+    fprintf(f, "%s %d %d %d %d %.6f %u\n", key, r.bs, r.opts.short_max, r.opts.group_lanes, r.opts.long_lanes,
+            r.best_ms, r.build_flags);
     fclose(f);
 }
 
@@ -602,25 +767,31 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     };
     float best = 1e30f;
     int32_t best_bs = 0;
+    uint32_t best_flags = 0;
     int tried = 0;
-    // stage 1: block size with the default thresholds
+    // stage 1: block size, with and without edge tables, default thresholds
     for (int32_t bs : cands) {
         if (bs <= 0 || bs > n) continue;
-        rmq_handle_t *h = nullptr;
-        if ((rc = rmq_build(d_A, n, bs, 0, s, &h)) != RMQ_OK) return rc;
-        float ms = 0.f;
-        rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 3, &ms);
-        rmq_free(h);
-        if (rc != RMQ_OK) return rc;
-        ++tried;
-        if (ms < best) {
-            best = ms;
-            best_bs = bs;
+        for (uint32_t flags : {0u, (uint32_t)RMQ_BUILD_EDGES}) {
+            rmq_handle_t *h = nullptr;
+            rc = rmq_build(d_A, n, bs, flags, s, &h);
+            if (rc == RMQ_ERR_OOM && flags) continue;  // no room for edge tables: plain only
+            if (rc != RMQ_OK) return rc;
+            float ms = 0.f;
+            rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 3, &ms);
+            rmq_free(h);
+            if (rc != RMQ_OK) return rc;
+            ++tried;
+            if (ms < best) {
+                best = ms;
+                best_bs = bs;
+                best_flags = flags;
+            }
         }
     }
     if (best_bs == 0) return RMQ_ERR_INVALID_ARG;
     rmq_handle_t *h = nullptr;
-    if ((rc = rmq_build(d_A, n, best_bs, 0, s, &h)) != RMQ_OK) return rc;
+    if ((rc = rmq_build(d_A, n, best_bs, best_flags, s, &h)) != RMQ_OK) return rc;
     rmq_query_opts_t best_o = base_opts(best_bs);
     // stage 2: short-range threshold and lanes per short query
     for (int32_t sm : {32, 128, 512, 1024, 4096}) {
@@ -654,6 +825,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     rmq_free(h);
     if (rc != RMQ_OK) return rc;
     out->bs = best_bs;
+    out->build_flags = best_flags;
     out->opts = best_o;
     out->best_ms = best;
     out->n_tried = tried;

@@ -208,6 +208,64 @@ __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// edge tables (RMQ_BUILD_EDGES): precomputed edge-block searches
+// ---------------------------------------------------------------------------
+constexpr int kSub = 32;  // sub-block size of the in-block level (elements)
+constexpr int kSubShift = 5;
+
+struct EdgeTables {
+    const Pair *pre_b;  // per element i: leftmost min of [start of i's block, i]
+    const Pair *suf_b;  // per element i: leftmost min of [i, end of i's block]
+    const Pair *pre_s;  // the same within i's 32-element sub-block (sub-blocks start
+    const Pair *suf_s;  //   at the block start; equal to pre_b / suf_b when bs <= 32)
+    const Pair *st_s;   // sparse table over sub-block minima: levels_s rows of subs_total
+    int64_t subs_total; // nb * nsub
+    int32_t nsub;       // sub-blocks per block, ceil(bs / 32)
+    int32_t levels_s;
+};
+
+// Leftmost minimum of the c >= 1 entries [a, a+c) of a sparse table whose
+// rows hold `stride` entries; the left window wins ties.
+__device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t stride, int64_t a, int c) {
+    const int lev = 31 - __clz(c);
+    const Pair *row = st + (int64_t)lev * stride;
+    const Pair x = row[a];
+    const Pair y = row[a + c - (1 << lev)];
+    return (y.v < x.v) ? y : x;
+}
+
+// O(1) answer of a range of more than 32 elements from the edge tables:
+// blocks differ -> suffix of the left block, whole blocks between (block
+// table), prefix of the right block; same block -> the same three parts one
+// level down (sub-blocks of the block; a range of > 32 elements spans at
+// least two).  Parts are combined left to right with strict '<', so the
+// leftmost minimum wins.  One lane per query: no reduction needed.
+__device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Pair *__restrict__ st, int nb, int bs,
+                                             uint64_t magic, int l, int r) {
+    const int bl = block_of(l, magic), br = block_of(r, magic);
+    Pair best, right;
+    if (bl != br) {
+        best = e.suf_b[l];
+        right = e.pre_b[r];
+        if (br - bl > 1) {
+            const Pair mid = table_min(st, nb, bl + 1, br - bl - 1);
+            if (mid.v < best.v) best = mid;
+        }
+    } else {
+        const int base = bl * bs;
+        const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
+        best = e.suf_s[l];
+        right = e.pre_s[r];
+        if (jr - jl > 1) {
+            const Pair mid = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl + 1, jr - jl - 1);
+            if (mid.v < best.v) best = mid;
+        }
+    }
+    if (right.v < best.v) best = right;
+    return best;
+}
+
 // Per-wave LDS staging of one tile's queries of a class.
 struct TileStage {
     int l[kWave];
@@ -314,6 +372,10 @@ struct rmq_handle {
     int device;
     rmqx::Pair *st;  // levels * nb pairs; level 0 = block minima
     int64_t st_bytes;
+    uint32_t flags;
+    rmqx::EdgeTables edges;  // all pointers null unless built with RMQ_BUILD_EDGES
+    void *edge_mem;          // one allocation holding every edge array
+    int64_t edge_bytes;
 };
 
 namespace rmqx {

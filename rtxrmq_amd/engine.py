@@ -69,12 +69,13 @@ class RMQIndex:
     The array tensor is kept referenced: the edge-block scans read it.
     """
 
-    def __init__(self, A, bs: int, stream=None):
+    def __init__(self, A, bs: int, flags: int = 0, stream=None):
         self.A = _dev_f32(A, "A")
         self.n = int(A.numel())
         self.bs = int(bs)
+        self.flags = int(flags)
         h = ctypes.c_void_p()
-        N.check(N.lib().rmq_build(A.data_ptr(), self.n, self.bs, 0, _stream_ptr(stream), ctypes.byref(h)),
+        N.check(N.lib().rmq_build(A.data_ptr(), self.n, self.bs, self.flags, _stream_ptr(stream), ctypes.byref(h)),
                 "rmq_build")
         self._h = h
 
@@ -174,7 +175,7 @@ def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool
     With a cache file, a pick stored for the same (n, length signature) is
     returned without timing anything; refresh=True times the sweep anyway and
     appends the new pick (the last line for a key wins).
-    Returns {"bs", "opts" (QueryOpts), "best_ms", "n_tried", "from_cache", "key"}.
+    Returns {"bs", "flags" (rmq_build flags), "opts" (QueryOpts), "best_ms", "n_tried", "from_cache", "key"}.
     """
     torch = _torch()
     A = _dev_f32(A, "A")
@@ -191,8 +192,10 @@ def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool
     o = res.opts
     if cache_path and refresh:  # same line format rmq_tune writes
         with open(cache_path, "a") as f:
-            f.write(f"{res.key.decode()} {res.bs} {o.short_max} {o.group_lanes} {o.long_lanes} {res.best_ms:.6f}\n")
-    return {"bs": res.bs, "opts": N.QueryOpts(o.short_max, o.group_lanes, o.long_wgs, o.long_lanes),
+            f.write(f"{res.key.decode()} {res.bs} {o.short_max} {o.group_lanes} {o.long_lanes} {res.best_ms:.6f} "
+                    f"{res.build_flags}\n")
+    return {"bs": res.bs, "flags": res.build_flags,
+            "opts": N.QueryOpts(o.short_max, o.group_lanes, o.long_wgs, o.long_lanes),
             "best_ms": res.best_ms, "n_tried": res.n_tried, "from_cache": bool(res.from_cache),
             "key": res.key.decode()}
 
@@ -293,10 +296,10 @@ class Server:
         self._arrays = []
         self._live = {}
 
-    def add_array(self, A, bs: int) -> int:
+    def add_array(self, A, bs: int, flags: int = 0) -> int:
         A = _dev_f32(A, "A")
         aid = ctypes.c_int32()
-        N.check(N.lib().rmq_server_add_array(self._h, A.data_ptr(), A.numel(), bs, ctypes.byref(aid)),
+        N.check(N.lib().rmq_server_add_array_ex(self._h, A.data_ptr(), A.numel(), bs, flags, ctypes.byref(aid)),
                 "rmq_server_add_array")
         self._arrays.append(A)
         return aid.value

@@ -39,6 +39,7 @@ def test_argument_validation_before_device_use():
     assert lib.rmq_build(fake, 10, 0, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(fake, 1 << 31, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(fake, 10, 4, 7, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_build(ctypes.c_void_p(0x1004), 10, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(fake, 10, 4, 0, None, None) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_query(None, fake, 4, fake, fake, None) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_gpu_base(None, 10, fake, 1, fake, None, None) == R.RMQ_ERR_INVALID_ARG

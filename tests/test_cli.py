@@ -1,6 +1,7 @@
 """The ./rtxrmq command line: same arguments, alg ids, options and CSV rows as
 the reference driver (src/main.cu:51-171, src/tools.h:28-189,268-336)."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -11,8 +12,14 @@ from rtxrmq_amd import _native as N
 CLI = os.path.join(N.PKG, "bin", "rtxrmq")
 
 
+ANSI = re.compile(r"\x1b\[[0-9;]*m")
+
+
 def run(*args, cwd=None):
-    return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+    r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+    r.stdout = ANSI.sub("", r.stdout)
+    r.stderr = ANSI.sub("", r.stderr)
+    return r
 
 
 @pytest.fixture(scope="module", autouse=True)

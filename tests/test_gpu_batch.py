@@ -50,7 +50,8 @@ def test_device_query_generator_matches_host(gpu):
 @pytest.mark.parametrize("opts", [None, R.query_opts(short_max=16, group_lanes=4, long_lanes=8),
                                   R.query_opts(short_max=1000, group_lanes=16, long_lanes=64)])
 @pytest.mark.parametrize("item_queries", [0, 97])
-def test_plan_matches_oracle(opts, item_queries, gpu):
+@pytest.mark.parametrize("flags", [0, R.BUILD_EDGES])
+def test_plan_matches_oracle(opts, item_queries, flags, gpu):
     arrays, queries = make_batch(gpu, 40, 4, 14, 11)
     exp = expected(arrays, queries)
     bss = [1, 3, 64, 256, 1000]
@@ -58,7 +59,7 @@ def test_plan_matches_oracle(opts, item_queries, gpu):
     Q_dev = [_dev(Q, gpu) for Q in queries]
     idx = [torch.full((Q.shape[0],), -7, dtype=torch.int32, device=gpu) for Q in queries]
     val = [torch.zeros(Q.shape[0], dtype=torch.float32, device=gpu) for Q in queries]
-    indexes = [R.RMQIndex(A, bss[i % len(bss)]) for i, A in enumerate(A_dev)]
+    indexes = [R.RMQIndex(A, bss[i % len(bss)], flags=flags if i % 2 else 0) for i, A in enumerate(A_dev)]
     with R.Plan(indexes, Q_dev, out_idx=idx, out_val=val, opts=opts, item_queries=item_queries) as plan:
         info = plan.info()
         assert info["arrays"] == 40 and info["queries"] == sum(Q.shape[0] for Q in queries)
@@ -71,11 +72,12 @@ def test_plan_matches_oracle(opts, item_queries, gpu):
         ix.close()
 
 
-def test_plan_equals_per_array_loop_and_graph_replays(gpu):
+@pytest.mark.parametrize("flags", [0, R.BUILD_EDGES])
+def test_plan_equals_per_array_loop_and_graph_replays(flags, gpu):
     arrays, queries = make_batch(gpu, 64, 6, 16, 23)
     A_dev = [_dev(A, gpu) for A in arrays]
     Q_dev = [_dev(Q, gpu) for Q in queries]
-    indexes = [R.RMQIndex(A, 256) for A in A_dev]
+    indexes = [R.RMQIndex(A, 256, flags=flags) for A in A_dev]
     loop = []
     for ix, Q in zip(indexes, Q_dev):
         loop.append(ix.query(Q, want_value=False, want_index=True)[1])

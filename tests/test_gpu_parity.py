@@ -21,10 +21,10 @@ def _dev(x, gpu):
     return torch.from_numpy(np.ascontiguousarray(x)).to(gpu)
 
 
-def run_engine(A_np, Q_np, bs, gpu, opts=None, A_dev=None):
+def run_engine(A_np, Q_np, bs, gpu, opts=None, A_dev=None, flags=0):
     A = A_dev if A_dev is not None else _dev(A_np, gpu)
     Q = _dev(Q_np.reshape(-1, 2), gpu)
-    with R.RMQIndex(A, bs) as ix:
+    with R.RMQIndex(A, bs, flags=flags) as ix:
         v, i = ix.query(Q, opts=opts)
         torch.cuda.synchronize()
         return v.cpu().numpy(), i.cpu().numpy()
@@ -36,23 +36,28 @@ def assert_exact(val, idx, ev, ei, what=""):
 
 
 # ---------------------------------------------------------------------------
+FLAGS = [0, R.BUILD_EDGES]
+
+
+@pytest.mark.parametrize("flags", FLAGS)
 @pytest.mark.parametrize("name", golden_cases())
-@pytest.mark.parametrize("bs", [1, 3, 4, 16, 1 << 15])
-def test_golden(name, bs, gpu):
+@pytest.mark.parametrize("bs", [1, 3, 4, 16, 33, 1 << 15])
+def test_golden(name, bs, flags, gpu):
     g = load_golden(name)
-    v, i = run_engine(g["A"], g["Q"], bs, gpu)
-    assert_exact(v, i, g["val"], g["idx"], f"{name} bs={bs}")
+    v, i = run_engine(g["A"], g["Q"], bs, gpu, flags=flags)
+    assert_exact(v, i, g["val"], g["idx"], f"{name} bs={bs} flags={flags}")
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 130, 257])
-@pytest.mark.parametrize("bs", [1, 2, 3, 5, 8, 64, 1000])
-def test_exhaustive_small(n, bs, gpu):
+@pytest.mark.parametrize("flags", FLAGS)
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 130, 257, 1000])
+@pytest.mark.parametrize("bs", [1, 2, 3, 5, 8, 31, 32, 33, 64, 100, 1000])
+def test_exhaustive_small(n, bs, flags, gpu):
     rng = np.random.default_rng(n * 7919 + bs)
     A = rng.integers(0, 4, n).astype(np.float32)
     Q = np.array([(l, r) for l in range(n) for r in range(l, n)], np.int32)
     ev, ei = O.rmq_brute(A, Q)
-    v, i = run_engine(A, Q, bs, gpu)
-    assert_exact(v, i, ev, ei, f"n={n} bs={bs}")
+    v, i = run_engine(A, Q, bs, gpu, flags=flags)
+    assert_exact(v, i, ev, ei, f"n={n} bs={bs} flags={flags}")
 
 
 @pytest.fixture(scope="module")
@@ -69,9 +74,9 @@ def mixed_case():
 def test_tuning_knobs_do_not_change_answers(group_lanes, long_lanes, short_max, mixed_case, gpu):
     A, Q, ev, ei = mixed_case
     opts = R.query_opts(short_max=short_max, group_lanes=group_lanes, long_lanes=long_lanes)
-    for bs in (37, 1000):
-        v, i = run_engine(A, Q, bs, gpu, opts=opts)
-        assert_exact(v, i, ev, ei, f"bs={bs} G={group_lanes} GL={long_lanes} short_max={short_max}")
+    for bs, flags in ((37, 0), (1000, 0), (37, R.BUILD_EDGES), (4096, R.BUILD_EDGES)):
+        v, i = run_engine(A, Q, bs, gpu, opts=opts, flags=flags)
+        assert_exact(v, i, ev, ei, f"bs={bs} flags={flags} G={group_lanes} GL={long_lanes} short_max={short_max}")
 
 
 def test_autotuner_pick_and_cache(tmp_path, gpu):
@@ -118,14 +123,15 @@ def test_config1_plain_small(gpu):
         np.testing.assert_array_equal(v.view(np.int32), hv.view(np.int32))
 
 
+@pytest.mark.parametrize("flags", FLAGS)
 @pytest.mark.parametrize("lr", [-1, -2, -3])
-def test_headline_full_size(lr, gpu):
+def test_headline_full_size(lr, flags, gpu):
     """n=1e8, q=2^26, bs=2^15, each length distribution: every answer checked."""
     n, q, bs = 100_000_000, 1 << 26, 1 << 15
     A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
     Q = R.gen_queries(q, n, lr, 1321 + lr)
     Q_dev = _dev(Q, gpu)
-    with R.RMQIndex(A_dev, bs) as ix:
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
         v, i = ix.query(Q_dev)
         torch.cuda.synchronize()
         v, i = v.cpu().numpy(), i.cpu().numpy()
@@ -139,22 +145,23 @@ def test_headline_full_size(lr, gpu):
     assert_exact(v, i, fv, fi, f"headline lr={lr}")
 
 
+@pytest.mark.parametrize("flags", FLAGS)
 @pytest.mark.parametrize("kind", ["ties16", "desc"])
-@pytest.mark.parametrize("bs", [1 << 10, 1 << 12, 1 << 15, 1 << 18])
-def test_stress_n2_27(kind, bs, gpu):
+@pytest.mark.parametrize("bs", [1 << 5, 1 << 10, 1 << 12, 1 << 15, 1 << 18])
+def test_stress_n2_27(kind, bs, flags, gpu):
     """n=2^27 heavy-tie and strictly descending arrays, fixed lengths
     {1, 2^8, 2^16, n-l}, block sizes across the sweep (2^20 queries per length)."""
     n = 1 << 27
     A_dev = R.gen_array(n, 99, kind, device=gpu)
     A = A_dev.cpu().numpy()
     f = O.FastOracle(A)
-    with R.RMQIndex(A_dev, bs) as ix:
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
         for lr in (1, 1 << 8, 1 << 16, -6):
             Q = R.gen_queries(1 << 20, n, lr, 7)
             v, i = ix.query(_dev(Q, gpu))
             torch.cuda.synchronize()
             fv, fi = f.query(Q)
-            assert_exact(v.cpu().numpy(), i.cpu().numpy(), fv, fi, f"{kind} bs={bs} lr={lr}")
+            assert_exact(v.cpu().numpy(), i.cpu().numpy(), fv, fi, f"{kind} bs={bs} flags={flags} lr={lr}")
     f.close()
 
 
@@ -181,9 +188,10 @@ def test_invalid_queries_are_flagged_not_faulted(gpu):
     assert R.check_queries(_dev(Q, gpu), 1000) == 3
 
 
-def test_empty_and_single(gpu):
+@pytest.mark.parametrize("flags", FLAGS)
+def test_empty_and_single(flags, gpu):
     A = _dev(np.array([2.5], np.float32), gpu)
-    with R.RMQIndex(A, 7) as ix:
+    with R.RMQIndex(A, 7, flags=flags) as ix:
         v, i = ix.query(torch.empty((0, 2), dtype=torch.int32, device=gpu))
         assert v.numel() == 0 and i.numel() == 0
         v, i = ix.query(_dev(np.array([[0, 0]], np.int32), gpu))

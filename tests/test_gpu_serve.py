@@ -40,9 +40,10 @@ def request_payloads(arrays, count, seed):
     return reqs
 
 
-def test_alone_and_in_the_mix(arrays, gpu):
+@pytest.mark.parametrize("flags", [0, R.BUILD_EDGES])
+def test_alone_and_in_the_mix(arrays, flags, gpu):
     with R.Server(slots=3) as sv:
-        ids = [sv.add_array(Ad, bs) for (_, Ad), bs in zip(arrays, (64, 1000, 256, 3))]
+        ids = [sv.add_array(Ad, bs, flags) for (_, Ad), bs in zip(arrays, (64, 1000, 256, 3))]
         reqs = request_payloads(arrays, 48, 5)
         # request 0 alone
         a0, Q0 = reqs[0]
@@ -102,3 +103,6 @@ def test_invalid_server_arguments(gpu):
     with R.Server() as sv:
         with pytest.raises(R.RMQError):
             sv.submit(99, np.zeros((4, 2), np.int32), out_idx=np.empty(4, np.int32))
+        A = torch.zeros(16, dtype=torch.float32, device=gpu)
+        with pytest.raises(R.RMQError):
+            sv.add_array(A, 4, 0x80)  # unknown build flag
