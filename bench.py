@@ -16,9 +16,14 @@ seconds) of the tuned run; answers are bit-identical either way.
 roofline: the dominant kernel of the run -- algorithmic bytes per query
 B = 8 (query) + 4*S (elements scanned: the range for a short or in-block
 query, else both edge-block parts) + T (two 8-byte table entries when whole
-blocks lie between) + 8 (value + index out), summed over the queries that
-kernel processed, divided by its time measured with HIP events on the stream
-it runs on.
+blocks lie between) + 8 (value + index out); with edge tables a long query
+scans nothing and reads two 8-byte edge pairs instead of 4*S.  Summed over
+the queries that kernel processed, divided by its time measured with HIP
+events on the stream it runs on.  `traffic` is the HBM bytes per launch from
+rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/traffic.json, written by
+--reduce-pmc), for exactly this workload or null.  An edge-table kernel is
+bound by random DRAM requests rather than bytes: `gather_ceiling` sets its
+random gathers per second against rmq_probe_gather's on the same GPU.
 
 cpu_baseline: the reference's HRMQ (RMQRMM64, built from the reference's own
 sources into oracle/_ref by oracle/Makefile) on this host's cores, OpenMP
@@ -74,19 +79,101 @@ def algorithmic_bytes(Q: np.ndarray, bs: int, short_max: int, out_bytes: int = 8
     return float(B[is_long].sum()), float(short_B[~is_long].sum()), int(is_long.sum()), int((~is_long).sum())
 
 
+TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def workload_key(n: int, q: int, bs_for: dict, opts_for: dict, edges_for: dict | None) -> str:
+    """Names a timed region exactly: HBM traffic measured for one configuration
+    is never reported for another."""
+    parts = []
+    for lr in DISTS:
+        o = opts_for[lr]
+        parts.append(f"{lr}:bs={bs_for[lr]}{'+edges' if edges_for and edges_for[lr] else ''}"
+                     f"/s{o.short_max}/g{o.group_lanes}/l{o.long_lanes}")
+    return f"n={n} q={q} value+index " + " ".join(parts)
+
+
 def load_traffic(kernel: str, workload: str):
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated)
+    measured for exactly this workload, else None."""
+    if not os.path.exists(TRAFFIC_PATH):
         return None
     try:
-        with open(path) as f:
-            d = json.load(f)
-        rec = d.get(kernel, {})
-        if rec.get("workload") == workload:
-            return rec.get("bytes_per_launch")
-    except (OSError, ValueError):
+        with open(TRAFFIC_PATH) as f:
+            rec = json.load(f).get(workload, {}).get(kernel)
+        return None if rec is None else rec["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
         return None
-    return None
+
+
+def _counter_rows(d: str, counter: str) -> list[tuple[str, float]]:
+    """(kernel name, value in KiB) per dispatch from a rocprofv3 --pmc output tree."""
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def reduce_pmc(d: str, copy_bytes: int) -> int:
+    """Turns four rocprofv3 --pmc runs under `d` into profiles/traffic.json.
+
+    d/fetch, d/write: FETCH_SIZE and WRITE_SIZE of a bench run whose JSON line
+    is d/bench.json; d/copy_fetch, d/copy_write: the same counters for
+    `bench.py --copy-probe copy_bytes` (each copy reads and writes copy_bytes).
+    Each counter is scaled by (bytes the copy moves) / (what the counter read
+    for it), then averaged over every dispatch of each query kernel."""
+    with open(os.path.join(d, "bench.json")) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    cal = {}
+    for c, sub in (("FETCH_SIZE", "copy_fetch"), ("WRITE_SIZE", "copy_write")):
+        rows = _counter_rows(os.path.join(d, sub), c)
+        kib = max(v for _, v in rows)  # the copy is the largest dispatch of the probe
+        cal[c] = copy_bytes / (kib * 1024.0)
+    per = {}
+    for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        for k, v in _counter_rows(os.path.join(d, sub), c):
+            per.setdefault(k, {}).setdefault(c, []).append(v * 1024.0 * cal[c])
+
+    def find(sym: str):
+        # "k_query_tiles<4, false>" matches the demangled "rmqx::k_query_tiles<4, false>(...)"
+        hits = [k for k in per if f"::{sym}(" in k or k.startswith(f"{sym}(")]
+        return per[hits[0]] if len(hits) == 1 else None
+
+    out = {}
+    if os.path.exists(TRAFFIC_PATH):
+        with open(TRAFFIC_PATH) as f:
+            out = json.load(f)
+    sections = [(line["roofline"]["workload"], line["per_kernel"])]
+    sections += [(v["roofline"]["workload"], v["per_kernel"]) for k, v in line.items() if k.startswith("fixed_bs_")]
+    seen = [sym for _, kernels in sections for sym in kernels]
+    for workload, kernels in sections:
+        for sym, info in kernels.items():
+            if seen.count(sym) > 1:  # launched by both runs: its dispatches cannot be told apart
+                log(f"traffic: {sym} runs in more than one section; not attributed")
+                continue
+            rec = find(sym)
+            if not rec or "FETCH_SIZE" not in rec or "WRITE_SIZE" not in rec:
+                continue
+            fb, wb = float(np.mean(rec["FETCH_SIZE"])), float(np.mean(rec["WRITE_SIZE"]))
+            out.setdefault(workload, {})[sym] = {
+                "bytes_per_launch": fb + wb, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                "dispatches": len(rec["FETCH_SIZE"]),
+                "algorithmic_bytes_per_launch": info["algorithmic_bytes_per_launch"],
+                "traffic_over_algorithmic": (fb + wb) / info["algorithmic_bytes_per_launch"],
+                "calibration": {"FETCH_SIZE": cal["FETCH_SIZE"], "WRITE_SIZE": cal["WRITE_SIZE"],
+                                "copy_bytes": copy_bytes}}
+            log(f"traffic: {sym}: {(fb + wb) / 1e6:.1f} MB/launch over {len(rec['FETCH_SIZE'])} dispatches, "
+                f"{(fb + wb) / info['algorithmic_bytes_per_launch']:.2f}x algorithmic")
+    with open(TRAFFIC_PATH, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
+    return 0
 
 
 def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | None:
@@ -160,7 +247,7 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
 def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -> dict:
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel."""
     per_dist = {}
-    kernels = {"k_query_long": [0.0, 0.0], "k_query_short": [0.0, 0.0]}  # bytes, ms
+    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches]
     for lr in DISTS:
         lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs_for[lr], opts_for[lr].short_max,
                                            edges=bool(edges_for and edges_for[lr]))
@@ -179,13 +266,26 @@ def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -
             "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
             "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
         }
-        kernels["k_query_long"][0] += lb * len(steps_out)
-        kernels["k_query_long"][1] += float(np.sum(lms))
-        kernels["k_query_short"][0] += sb * len(steps_out)
-        kernels["k_query_short"][1] += float(np.sum(sms))
+        long_sym = ("k_query_edges" if edges_for and edges_for[lr]
+                    else f"k_query_tiles<{opts_for[lr].long_lanes}, true>")
+        short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
+        per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
+        gathers = 2 * nl if edges_for and edges_for[lr] else 0  # suffix pair at l, prefix pair at r
+        for sym, b, t, g in ((long_sym, lb, lms, gathers), (short_sym, sb, sms, 0)):
+            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0])
+            acc[0] += b * len(steps_out)
+            acc[1] += float(np.sum(t))
+            acc[2] += len(steps_out)
+            acc[3] += g * len(steps_out)
     dom = max(kernels, key=lambda k: kernels[k][1])
     achieved = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
-    per_kernel = {k: {"ms": v[1], "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
+    per_kernel = {k: {"ms": v[1], "launches": v[2], "avg_launch_ms": v[1] / v[2],
+                      "algorithmic_bytes_per_launch": v[0] / v[2],
+                      "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
+    for k, v in kernels.items():
+        if v[3]:
+            per_kernel[k]["random_gathers_per_launch"] = v[3] / v[2]
+            per_kernel[k]["random_gathers_per_s"] = v[3] / (v[1] / 1e3)
     return {"value": 3 * q * len(steps_out) / elapsed, "ms_per_step": elapsed / len(steps_out) * 1e3,
             "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
 
@@ -451,7 +551,14 @@ def main() -> int:
                     help="run one section alone and print its JSON object (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
+    ap.add_argument("--no-gather-probe", action="store_true", help="skip the random-gather ceiling probe")
+    ap.add_argument("--reduce-pmc", default=None, metavar="DIR",
+                    help="CPU only: turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into profiles/traffic.json")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size the --copy-probe run used")
     args = ap.parse_args()
+
+    if args.reduce_pmc:
+        return reduce_pmc(args.reduce_pmc, args.copy_bytes)
 
     if args.copy_probe:
         import torch
@@ -531,8 +638,19 @@ def main() -> int:
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
     for ix in indexes.values():
         ix.close()
+    gather_ceiling = None
+    if "random_gathers_per_s" in tuned["per_kernel"][tuned["dominant"]] and not args.no_gather_probe:
+        gp = R.probe_gather(device=dev)
+        k_rate = tuned["per_kernel"][tuned["dominant"]]["random_gathers_per_s"]
+        gather_ceiling = {"gathers_per_s": gp["gathers_per_s"], "kernel_gathers_per_s": k_rate,
+                          "frac": k_rate / gp["gathers_per_s"],
+                          "probe": "rmq_probe_gather: 2^26 threads x 2 random 8 B gathers from two 1 GiB tables"}
+        log(f"bench: random-gather ceiling {gp['gathers_per_s']:.3e}/s; {tuned['dominant']} "
+            f"{k_rate:.3e}/s ({k_rate / gp['gathers_per_s']:.3f} of it)")
 
-    workload = f"n={n} q={q} lr=-1,-2,-3 value+index tuned"
+    edges_for = {lr: picks[lr]["flags"] for lr in DISTS}
+    workload = workload_key(n, q, bs_for, opts_for, edges_for)
+    fixed_workload = workload_key(n, q, {lr: args.bs for lr in DISTS}, {lr: fixed_opts for lr in DISTS}, None)
     del out_v, out_i
     batched = None if args.no_batched else run_batched(R, torch, dev, args)
     served = None if args.no_served else run_served(R, torch, dev, args)
@@ -565,7 +683,11 @@ def main() -> int:
                    "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},
         "roofline": {"bound": "hbm", "achieved": tuned["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tuned["achieved"] / HBM_PEAK_GBS,
-                     "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"]},
+                     "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"],
+                     "avg_launch_ms": tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"],
+                     "algorithmic_bytes_per_launch":
+                         tuned["per_kernel"][tuned["dominant"]]["algorithmic_bytes_per_launch"],
+                     "workload": workload, "gather_ceiling": gather_ceiling},
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
@@ -574,9 +696,12 @@ def main() -> int:
         f"fixed_bs_{args.bs}": {
             "value": fixed["value"], "ms_per_step": fixed["ms_per_step"], "steps": args.fixed_steps,
             "construction_ms": fixed_build_ms, "per_dist": fixed["per_dist"],
+            "per_kernel": fixed["per_kernel"],
             "roofline": {"kernel": fixed["dominant"], "achieved": fixed["achieved"],
                          "frac": fixed["achieved"] / HBM_PEAK_GBS,
-                         "traffic": load_traffic(fixed["dominant"], f"n={n} q={q} bs={args.bs} lr=-1,-2,-3 value+index")}},
+                         "traffic": load_traffic(fixed["dominant"], fixed_workload),
+                         "avg_launch_ms": fixed["per_kernel"][fixed["dominant"]]["avg_launch_ms"],
+                         "workload": fixed_workload}},
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -181,13 +181,17 @@ int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t 
  *        it returns at once with a ticket.  The buffers must stay valid until
  *        rmq_server_wait returns for that ticket.
  *      - A scheduler thread admits pending requests FIFO while their device
- *        buffers fit in the memory budget (free HBM at creation less a
- *        reserve, or mem_budget_bytes if smaller) and coalesces them, up to
+ *        buffers fit in the memory budget and coalesces them, up to
  *        max_batch_queries, into one plan launch on a free slot (stream):
  *        upload, one persistent kernel over all their work items, download.
  *        Up to `slots` launches are in flight; requests arriving meanwhile go
  *        to the next free slot, so their work overlaps on the device.
  *      - Latency is measured from submit to the answers in host memory.
+ *      - Memory budget: mem_budget_bytes when set; its unspent part stays
+ *        free, so an array whose tables do not fit beside it is refused
+ *        (RMQ_ERR_OOM).  Unset, the budget is free HBM less a runtime reserve
+ *        (a quarter of free HBM, at most 8 GiB), and it shrinks as resident
+ *        arrays are added, so that the arrays' tables take precedence.
  *      Answers are the same as rmq_query's for the same array and queries,
  *      whatever else shares the launch. ---- */
 typedef struct rmq_server rmq_server_t;
@@ -256,6 +260,15 @@ int rmq_gen_queries(int32_t *d_Q, int64_t q, int32_t n, int32_t lr, uint64_t see
 int rmq_gen_queries_ref(int32_t *Q, int64_t q, int32_t n, int32_t lr, int32_t seed, int32_t nt);
 
 /* Static description of a return code. */
+/* Measurement (no counterpart in the reference): the random-request ceiling
+ * that bounds the edge-table query kernel.  count threads each read two
+ * 8-byte entries at hashed positions of d_t0 and d_t1 (entries each, a power
+ * of two) and write 8 bytes to d_out (count entries); one warm-up launch,
+ * then reps timed launches on s; *ms_per_launch is their mean.  Random
+ * gathers per second = 2 * count / (ms_per_launch / 1e3). */
+int rmq_probe_gather(const void *d_t0, const void *d_t1, int64_t entries, int64_t count, void *d_out, int32_t reps,
+                     hipStream_t s, float *ms_per_launch);
+
 const char *rmq_strerror(int code);
 
 /* Library version string. */

@@ -9,12 +9,12 @@ from ._native import (BUILD_EDGES, GEN_KINDS, RMQ_ERR_BAD_QUERY, RMQ_ERR_DEVICE,
 from .engine import (ALG_CPU_BASE, ALG_CPU_BASE_IDX, ALG_CPU_HRMQ, ALG_CPU_HRMQ_IDX, ALG_GPU_BASE,
                      ALG_GPU_BASE_IDX, ALG_GPU_RTX_BLOCKS, ALG_GPU_RTX_BLOCKS_IDX, RMQIndex, RunResult,
                      Plan, Server, check_queries, gen_array, gen_array_host, gen_queries, gen_queries_device, gen_queries_ref, gpu_base,
-                     query_opts, rtx_rmq, tune)
+                     probe_gather, query_opts, rtx_rmq, tune)
 
 __all__ = [
     "BUILD_EDGES", "GEN_KINDS", "RMQ_OK", "RMQ_ERR_INVALID_ARG", "RMQ_ERR_OOM", "RMQ_ERR_HIP", "RMQ_ERR_DEVICE",
     "RMQ_ERR_BAD_QUERY", "QueryOpts", "QueryTimes", "RMQError", "check", "header_functions", "lib", "strerror",
     "ALG_CPU_BASE", "ALG_CPU_BASE_IDX", "ALG_CPU_HRMQ", "ALG_CPU_HRMQ_IDX", "ALG_GPU_BASE", "ALG_GPU_BASE_IDX",
     "ALG_GPU_RTX_BLOCKS", "ALG_GPU_RTX_BLOCKS_IDX", "RMQIndex", "RunResult", "check_queries", "gen_array",
-    "gen_array_host", "gen_queries", "gen_queries_device", "gen_queries_ref", "gpu_base", "query_opts", "rtx_rmq", "tune", "Plan", "Server",
+    "gen_array_host", "gen_queries", "gen_queries_device", "gen_queries_ref", "gpu_base", "probe_gather", "query_opts", "rtx_rmq", "tune", "Plan", "Server",
 ]

@@ -108,6 +108,7 @@ def _declare(lib):
         "rmq_server_wait": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_double)]),
         "rmq_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats), _i32]),
         "rmq_server_destroy": (None, [_vp]),
+        "rmq_probe_gather": (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _i32, _vp, ctypes.POINTER(ctypes.c_float)]),
         "rmq_strerror": (ctypes.c_char_p, [ctypes.c_int]),
         "rmq_version": (ctypes.c_char_p, []),
     }

@@ -366,6 +366,8 @@ struct rmq_server {
     std::unordered_map<uint64_t, Request> reqs;
     uint64_t next_ticket = 1;
     int64_t budget = 0, inflight = 0, peak_inflight = 0;
+    int64_t reserve = 0;        // HBM kept free for the runtime
+    bool budget_fixed = false;  // the caller set mem_budget_bytes
     bool stop = false;
     std::thread sched;
     std::vector<std::thread> completers;
@@ -627,9 +629,10 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
     if (rc == RMQ_OK) {
         // admission budget for request buffers: free HBM less a reserve for
         // array tables and the runtime, or the configured cap if smaller
-        const int64_t reserve = (int64_t)std::min<size_t>(free_b / 4, (size_t)8 << 30);
-        sv->budget = (int64_t)free_b - reserve;
+        sv->reserve = (int64_t)std::min<size_t>(free_b / 4, (size_t)8 << 30);
+        sv->budget = (int64_t)free_b - sv->reserve;
         if (c.mem_budget_bytes > 0) sv->budget = std::min<int64_t>(sv->budget, c.mem_budget_bytes);
+        sv->budget_fixed = c.mem_budget_bytes > 0;
         sv->kernel = pick_kernel(c.opts.group_lanes, c.opts.long_lanes);
         rc = hip_rc(hipMalloc(&sv->d_descs, sizeof(ArrayDesc) * (size_t)c.max_arrays));
         sv->descs_cap = c.max_arrays;
@@ -670,8 +673,11 @@ RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t 
     size_t free_b = 0, total_b = 0;
     int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
     if (rc != RMQ_OK) return rc;
-    // the memory promised to request buffers (budget not yet in flight) stays free
-    if (table + (sv->budget - sv->inflight) > (int64_t)free_b) return RMQ_ERR_OOM;
+    // A configured budget is a promise: its unspent part stays free.  A budget
+    // derived from free HBM yields to resident arrays instead (it shrinks
+    // below, once the tables are built); only the runtime reserve stays free.
+    const int64_t promised = sv->budget_fixed ? sv->budget - sv->inflight : sv->reserve;
+    if (table + promised > (int64_t)free_b) return RMQ_ERR_OOM;
     rmq_handle_t *h = nullptr;
     hipStream_t s = sv->slots[0].stream;
     rc = rmq_build(d_A, n, bs, flags, s, &h);
@@ -690,6 +696,8 @@ RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t 
     }
     sv->arrays.push_back(h);
     *array_id = id;
+    if (!sv->budget_fixed && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        sv->budget = std::max<int64_t>(0, std::min<int64_t>(sv->budget, sv->inflight + (int64_t)free_b - sv->reserve));
     return RMQ_OK;
 }
 

@@ -362,6 +362,19 @@ int query_impl(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val,
     return rc;
 }
 
+// Random-request ceiling: each thread reads two 8-byte entries at hashed
+// positions of two tables (masked to the table size, so always in bounds)
+// and writes one 8-byte result -- the access pattern of a long range answered
+// from edge tables, with no RMQ logic around it.
+__global__ void k_probe_gather(const uint2 *__restrict__ t0, const uint2 *__restrict__ t1, uint64_t mask, int64_t m,
+                               uint2 *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t h0 = mix64((uint64_t)i), h1 = mix64(h0);
+        const uint2 a = t0[h0 & mask], b = t1[h1 & mask];
+        out[i] = make_uint2(a.x ^ b.x, a.y + b.y);
+    }
+}
+
 }  // namespace rmqx
 
 using namespace rmqx;
@@ -897,6 +910,33 @@ RMQ_API int rmq_gen_queries_ref(int32_t *Q, int64_t q, int32_t n, int32_t lr, in
     fill(0);
     for (auto &th : pool) th.join();
     return RMQ_OK;
+}
+
+RMQ_API int rmq_probe_gather(const void *d_t0, const void *d_t1, int64_t entries, int64_t count, void *d_out,
+                             int32_t reps, hipStream_t s, float *ms_per_launch) {
+    if (!d_t0 || !d_t1 || !d_out || !ms_per_launch || entries < 2 || (entries & (entries - 1)) || count <= 0 ||
+        reps <= 0)
+        return RMQ_ERR_INVALID_ARG;
+    const int grid = device_cus() * 16;
+    auto launch = [&]() {
+        k_probe_gather<<<grid, kWG, 0, s>>>(static_cast<const uint2 *>(d_t0), static_cast<const uint2 *>(d_t1),
+                                            (uint64_t)entries - 1, count, static_cast<uint2 *>(d_out));
+        return hip_rc(hipGetLastError());
+    };
+    int rc = launch();  // warm-up
+    hipEvent_t a = nullptr, b = nullptr;
+    if (rc == RMQ_OK) rc = hip_rc(hipEventCreate(&a));
+    if (rc == RMQ_OK) rc = hip_rc(hipEventCreate(&b));
+    if (rc == RMQ_OK) rc = hip_rc(hipEventRecord(a, s));
+    for (int r = 0; r < reps && rc == RMQ_OK; ++r) rc = launch();
+    if (rc == RMQ_OK) rc = hip_rc(hipEventRecord(b, s));
+    if (rc == RMQ_OK) rc = hip_rc(hipEventSynchronize(b));
+    float ms = 0.f;
+    if (rc == RMQ_OK) rc = hip_rc(hipEventElapsedTime(&ms, a, b));
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc == RMQ_OK) *ms_per_launch = ms / (float)reps;
+    return rc;
 }
 
 RMQ_API const char *rmq_strerror(int code) {

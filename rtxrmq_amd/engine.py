@@ -215,6 +215,26 @@ def gen_queries_ref(q: int, n: int, lr: int, seed: int, nt: int) -> np.ndarray:
     return Q
 
 
+def probe_gather(table_bytes: int = 1 << 30, count: int = 1 << 26, reps: int = 5, device="cuda",
+                 stream=None) -> dict:
+    """Random-request ceiling of this GPU (rmq_probe_gather): count threads
+    each gather two 8-byte entries at hashed positions of two tables of
+    table_bytes each.  Returns the launch time and random gathers per second."""
+    torch = _torch()
+    entries = table_bytes // 8
+    if entries < 2 or entries & (entries - 1):
+        raise ValueError("table_bytes must be 8 times a power of two")
+    t0 = torch.zeros(entries, dtype=torch.int64, device=device)
+    t1 = torch.ones(entries, dtype=torch.int64, device=device)
+    out = torch.empty(count, dtype=torch.int64, device=device)
+    ms = ctypes.c_float()
+    N.check(N.lib().rmq_probe_gather(t0.data_ptr(), t1.data_ptr(), entries, count, out.data_ptr(), reps,
+                                     _stream_ptr(stream), ctypes.byref(ms)), "rmq_probe_gather")
+    del t0, t1, out
+    return {"ms_per_launch": ms.value, "gathers_per_s": 2 * count / (ms.value / 1e3), "table_bytes": table_bytes,
+            "count": count, "reps": reps}
+
+
 def gen_queries_device(q: int, n: int, lr: int, seed: int, device="cuda", stream=None):
     """Device queries from the same counter-based stream (rmq_gen_queries); (q, 2) int32 tensor."""
     torch = _torch()

@@ -34,6 +34,10 @@ def test_argument_validation_before_device_use():
     lib = R.lib()
     h = ctypes.c_void_p()
     fake = ctypes.c_void_p(0x1000)
+    ms = ctypes.c_float()
+    for args in ((None, fake, 1 << 10, 64, fake, 1), (fake, fake, 1000, 64, fake, 1), (fake, fake, 1, 64, fake, 1),
+                 (fake, fake, 1 << 10, 0, fake, 1), (fake, fake, 1 << 10, 64, None, 1), (fake, fake, 1 << 10, 64, fake, 0)):
+        assert lib.rmq_probe_gather(*args, None, ctypes.byref(ms)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(None, 10, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(fake, 0, 4, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_build(fake, 10, 0, 0, None, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG

@@ -106,3 +106,15 @@ def test_invalid_server_arguments(gpu):
         A = torch.zeros(16, dtype=torch.float32, device=gpu)
         with pytest.raises(R.RMQError):
             sv.add_array(A, 4, 0x80)  # unknown build flag
+
+
+def test_derived_budget_yields_to_resident_arrays(arrays, gpu):
+    with R.Server(slots=1) as sv:
+        b0 = sv.stats()["budget_bytes"]
+        aid = sv.add_array(arrays[2][1], 64, R.BUILD_EDGES)  # ~40 MB of tables
+        b1 = sv.stats()["budget_bytes"]
+        Q = O.gen_queries(5000, arrays[2][0].size, -1, 3)
+        out = np.empty(5000, np.int32)
+        assert sv.wait(sv.submit(aid, Q, out_idx=out))[0] == R.RMQ_OK
+    assert 0 < b1 < b0
+    np.testing.assert_array_equal(out, O.rmq_brute(arrays[2][0], Q)[1])
