@@ -56,6 +56,9 @@ struct Args {
     int short_max = 0, group_lanes = 0, long_lanes = 0;
     int array_kind = RMQ_GEN_UNIFORM;
     bool qgen_ref = false;  // the reference's own query stream (src/rand.cuh:116-201)
+    bool edges = false;     // build edge tables (RMQ_BUILD_EDGES)
+    bool tune = false;      // pick bs, edge tables and thresholds with rmq_tune
+    std::string tune_cache;
 };
 
 void usage() {
@@ -82,6 +85,9 @@ void usage() {
             "  --short-max <len>     ranges up to this length use the group scan (alg 5/105)\n"
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
+            "  --edges               build edge tables: O(1) answers for ranges over 32 elements\n"
+            "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^22 queries\n"
+            "                        (overrides --bs/--nb/--edges and the lane options; optional pick cache)\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
             "  --qgen <gen>          ours (default: counter-based, thread-count independent) |\n"
             "                        ref (the reference's mt19937(seed*tid) stream; depends on --nt;\n"
@@ -110,7 +116,8 @@ bool parse(int argc, char **argv, Args &a) {
     a.bs = std::min(a.n, 1 << 15);
     a.nb = a.n / a.bs;
     a.seed = (int)time(nullptr);
-    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG, O_QGEN };
+    enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG, O_QGEN,
+           O_EDGES, O_TUNE };
     static struct option opts[] = {
         {"bs", required_argument, nullptr, O_BS},         {"nb", required_argument, nullptr, O_NB},
         {"reps", required_argument, nullptr, O_REPS},     {"dev", required_argument, nullptr, O_DEV},
@@ -119,7 +126,8 @@ bool parse(int argc, char **argv, Args &a) {
         {"save-power", optional_argument, nullptr, O_POWER},
         {"short-max", required_argument, nullptr, O_SHORT}, {"group-lanes", required_argument, nullptr, O_GROUP},
         {"array", required_argument, nullptr, O_ARRAY},   {"long-lanes", required_argument, nullptr, O_LONG},
-        {"qgen", required_argument, nullptr, O_QGEN},     {nullptr, 0, nullptr, 0}};
+        {"qgen", required_argument, nullptr, O_QGEN},     {"edges", no_argument, nullptr, O_EDGES},
+        {"tune", optional_argument, nullptr, O_TUNE},     {nullptr, 0, nullptr, 0}};
     int opt, idx;
     optind = 5;
     while ((opt = getopt_long(argc, argv, "", opts, &idx)) != -1) {
@@ -136,6 +144,8 @@ bool parse(int argc, char **argv, Args &a) {
         case O_SHORT: a.short_max = atoi(optarg); break;
         case O_GROUP: a.group_lanes = atoi(optarg); break;
         case O_LONG: a.long_lanes = atoi(optarg); break;
+        case O_EDGES: a.edges = true; break;
+        case O_TUNE: a.tune = true; if (optarg) a.tune_cache = optarg; break;
         case O_QGEN:
             if (!strcmp(optarg, "ref")) a.qgen_ref = true;
             else if (strcmp(optarg, "ours")) return false;
@@ -313,6 +323,33 @@ int main(int argc, char **argv) {
     }
     printf("done: %f secs\n" C_RESET, (now_ms() - t0) / 1000.0);
 
+    const bool engine = a.alg == ALG_GPU_RTX_BLOCKS || a.alg == ALG_GPU_RTX_BLOCKS_IDX;
+    rmq_query_opts_t opts{a.short_max, a.group_lanes, 0, a.long_lanes};
+    if (engine && a.tune) {
+        // timed passes over a prefix of the queries; answers do not depend on the pick
+        const int64_t sample = std::min<int64_t>(a.q, (int64_t)1 << 22);
+        float *d_sv = nullptr;
+        int32_t *d_si = nullptr;
+        hip_ok(hipMalloc(&d_sv, sizeof(float) * (size_t)sample), "hipMalloc(tune)");
+        hip_ok(hipMalloc(&d_si, sizeof(int32_t) * (size_t)sample), "hipMalloc(tune)");
+        printf("Autotuning on %lld queries.................", (long long)sample);
+        fflush(stdout);
+        t0 = now_ms();
+        rmq_tune_result_t pick{};
+        if (int rc = rmq_tune(dA, a.n, dQ, sample, nullptr, 0, idx_out ? nullptr : d_sv, idx_out ? d_si : nullptr,
+                              a.tune_cache.empty() ? nullptr : a.tune_cache.c_str(), nullptr, &pick))
+            die("rmq_tune", rc);
+        (void)hipFree(d_sv);
+        (void)hipFree(d_si);
+        a.bs = pick.bs;
+        a.nb = a.n / a.bs;
+        a.edges = (pick.build_flags & RMQ_BUILD_EDGES) != 0;
+        opts = pick.opts;
+        printf("done: %f secs [bs=%d edges=%d short_max=%d group_lanes=%d long_lanes=%d; %s]\n",
+               (now_ms() - t0) / 1000.0, a.bs, (int)a.edges, opts.short_max, opts.group_lanes, opts.long_lanes,
+               pick.from_cache ? "cached" : "timed");
+    }
+
     csv_prefix(a);
 
     // 2) computation
@@ -335,13 +372,11 @@ int main(int argc, char **argv) {
         if (idx_out) hip_ok(hipMalloc(&d_idx, sizeof(int32_t) * (size_t)a.q), "hipMalloc(out)");
         else hip_ok(hipMalloc(&d_val, sizeof(float) * (size_t)a.q), "hipMalloc(out)");
         rmq_handle_t *h = nullptr;
-        rmq_query_opts_t opts{a.short_max, a.group_lanes, 0, a.long_lanes};
-        const bool engine = a.alg == ALG_GPU_RTX_BLOCKS || a.alg == ALG_GPU_RTX_BLOCKS_IDX;
         if (engine) {
-            printf("Building block index (bs=%i)..............", a.bs);
+            printf("Building block index (bs=%i%s)........", a.bs, a.edges ? ", edge tables" : "");
             fflush(stdout);
             t0 = now_ms();
-            if (int rc = rmq_build(dA, a.n, a.bs, 0, nullptr, &h)) die("rmq_build", rc);
+            if (int rc = rmq_build(dA, a.n, a.bs, a.edges ? RMQ_BUILD_EDGES : 0u, nullptr, &h)) die("rmq_build", rc);
             hip_ok(hipDeviceSynchronize(), "build");
             construction_ms = now_ms() - t0;
             int64_t nb = 0, bytes = 0;

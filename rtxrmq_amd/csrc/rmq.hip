@@ -76,58 +76,98 @@ __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__
     cur[i] = (y.v < x.v) ? y : x;
 }
 
-// Edge tables, step 1: one thread per sub-block (32 elements, aligned to its
-// block's start): leftmost prefix and suffix (min, argmin) inside it and its
-// minimum.  Prefix: strict '<' going right; suffix: '<=' going left, so both
-// keep the leftmost of equal values.  Sub-blocks past a partial last block
-// are empty and get the sentinel (they are never read by a valid query).
-__global__ void k_sub_edges(const float *__restrict__ A, int n, int bs, int nsub, int64_t subs_total,
-                            Pair *__restrict__ pre_s, Pair *__restrict__ suf_s, Pair *__restrict__ submin) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Edge tables, step 1: 32 lanes per sub-block (32 elements, aligned to its
+// block's start), one lane per element, so loads and stores are contiguous.
+// Segmented inclusive scans across the 32 lanes give the prefix and suffix
+// (min, argmin) inside the sub-block; the lexicographic (value, index) order
+// keeps the leftmost of equal values in both directions.  Lane 0's suffix is
+// the sub-block minimum.  Sub-blocks past a partial last block are empty and
+// get the sentinel (they are never read by a valid query).
+__global__ __launch_bounds__(kWG) void k_sub_edges(const float *__restrict__ A, int n, int bs, int nsub,
+                                                   int64_t subs_total, Pair *__restrict__ pre_s,
+                                                   Pair *__restrict__ suf_s, Pair *__restrict__ submin) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g = t >> kSubShift;  // the same for all 32 lanes of a segment
     if (g >= subs_total) return;
+    const int o = (int)(t & (kSub - 1));
     const int64_t b = g / nsub, j = g - b * nsub;
     const int64_t start = b * bs + j * kSub;
     const int64_t end = min(min(start + kSub, (b + 1) * bs), (int64_t)n);
-    if (start >= end) {
-        submin[g] = Pair{INFINITY, INT_MAX};
-        return;
+    const int64_t i = start + o;
+    const bool valid = i < end;
+    const float x = valid ? A[i] : INFINITY;
+    const int xi = valid ? (int)i : INT_MAX;
+    float pv = x, sv = x;
+    int pi = xi, si = xi;
+#pragma unroll
+    for (int d = 1; d < kSub; d <<= 1) {
+        const float uv = __shfl_up(pv, d, kSub);
+        const int ui = __shfl_up(pi, d, kSub);
+        const float dv = __shfl_down(sv, d, kSub);
+        const int di = __shfl_down(si, d, kSub);
+        if (o >= d) take_if_better(uv, ui, pv, pi);
+        if (o + d < kSub) take_if_better(dv, di, sv, si);
     }
-    Pair m{A[start], (int32_t)start};
-    for (int64_t i = start; i < end; ++i) {
-        const float x = A[i];
-        if (x < m.v) m = Pair{x, (int32_t)i};
-        if (pre_s) pre_s[i] = m;
+    if (valid) {
+        if (pre_s) pre_s[i] = Pair{pv, pi};
+        if (suf_s) suf_s[i] = Pair{sv, si};
     }
-    submin[g] = m;
-    if (!suf_s) return;
-    m = Pair{A[end - 1], (int32_t)(end - 1)};
-    for (int64_t i = end - 1; i >= start; --i) {
-        const float x = A[i];
-        if (!(m.v < x)) m = Pair{x, (int32_t)i};
-        suf_s[i] = m;
-    }
+    if (o == 0) submin[g] = Pair{sv, si};
 }
 
-// Step 2: one thread per block: exclusive prefix and suffix minima over the
-// block's sub-block minima (index tie-break, so the leftmost wins).
-__global__ void k_block_sub_scan(const Pair *__restrict__ submin, int nb, int nsub, Pair *__restrict__ excl_pre,
-                                 Pair *__restrict__ excl_suf) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const int64_t base = (int64_t)b * nsub;
-    float rv = INFINITY;
-    int ri = INT_MAX;
-    for (int j = 0; j < nsub; ++j) {
-        excl_pre[base + j] = Pair{rv, ri};
-        const Pair x = submin[base + j];
-        take_if_better(x.v, x.i, rv, ri);
+// Step 2: one wave per block: exclusive prefix and suffix minima over the
+// block's sub-block minima, 64 sub-blocks per step with a carry across steps.
+__global__ __launch_bounds__(kWG) void k_block_sub_scan(const Pair *__restrict__ submin, int nb, int nsub,
+                                                        Pair *__restrict__ excl_pre, Pair *__restrict__ excl_suf) {
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // uniform across the wave
+    if (w >= nb) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t base = w * nsub;
+    float cv = INFINITY;  // carry: minimum of the sub-blocks already passed
+    int ci = INT_MAX;
+    for (int c = 0; c < nsub; c += kWave) {
+        const int j = c + lane;
+        const Pair x = j < nsub ? submin[base + j] : Pair{INFINITY, INT_MAX};
+        float v = x.v;
+        int i = x.i;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const float ov = __shfl_up(v, d);
+            const int oi = __shfl_up(i, d);
+            if (lane >= d) take_if_better(ov, oi, v, i);
+        }
+        float ev = __shfl_up(v, 1);
+        int ei = __shfl_up(i, 1);
+        if (lane == 0) {
+            ev = INFINITY;
+            ei = INT_MAX;
+        }
+        take_if_better(cv, ci, ev, ei);
+        if (j < nsub) excl_pre[base + j] = Pair{ev, ei};
+        take_if_better(__shfl(v, kWave - 1), __shfl(i, kWave - 1), cv, ci);
     }
-    rv = INFINITY;
-    ri = INT_MAX;
-    for (int j = nsub - 1; j >= 0; --j) {
-        excl_suf[base + j] = Pair{rv, ri};
-        const Pair x = submin[base + j];
-        take_if_better(x.v, x.i, rv, ri);
+    cv = INFINITY;
+    ci = INT_MAX;
+    for (int c = (nsub - 1) / kWave * kWave; c >= 0; c -= kWave) {
+        const int j = c + lane;
+        const Pair x = j < nsub ? submin[base + j] : Pair{INFINITY, INT_MAX};
+        float v = x.v;
+        int i = x.i;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const float ov = __shfl_down(v, d);
+            const int oi = __shfl_down(i, d);
+            if (lane + d < kWave) take_if_better(ov, oi, v, i);
+        }
+        float ev = __shfl_down(v, 1);
+        int ei = __shfl_down(i, 1);
+        if (lane == kWave - 1) {
+            ev = INFINITY;
+            ei = INT_MAX;
+        }
+        take_if_better(cv, ci, ev, ei);
+        if (j < nsub) excl_suf[base + j] = Pair{ev, ei};
+        take_if_better(__shfl(v, 0), __shfl(i, 0), cv, ci);
     }
 }
 
@@ -408,7 +448,8 @@ int build_edges(rmq_handle *h, hipStream_t s) {
     Pair *st_s = p + per_elem * n;
     Pair *excl_pre = st_s + (int64_t)levels_s * subs;
     Pair *excl_suf = excl_pre + subs;
-    k_sub_edges<<<(unsigned)((subs + 255) / 256), 256, 0, s>>>(h->A, (int)n, bs, nsub, subs, pre_s, suf_s, st_s);
+    k_sub_edges<<<(unsigned)((subs * kSub + kWG - 1) / kWG), kWG, 0, s>>>(h->A, (int)n, bs, nsub, subs, pre_s, suf_s,
+                                                                         st_s);
     rc = hip_rc(hipGetLastError());
     for (int k = 1; rc == RMQ_OK && k < levels_s; ++k) {
         const int64_t count = subs - ((int64_t)1 << k) + 1;
@@ -418,7 +459,8 @@ int build_edges(rmq_handle *h, hipStream_t s) {
         rc = hip_rc(hipGetLastError());
     }
     if (rc == RMQ_OK && two_level) {
-        k_block_sub_scan<<<(unsigned)((h->nb + 255) / 256), 256, 0, s>>>(st_s, (int)h->nb, nsub, excl_pre, excl_suf);
+        k_block_sub_scan<<<(unsigned)((h->nb + kWavesPerWG - 1) / kWavesPerWG), kWG, 0, s>>>(st_s, (int)h->nb, nsub,
+                                                                                           excl_pre, excl_suf);
         rc = hip_rc(hipGetLastError());
         if (rc == RMQ_OK) {
             k_block_edges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, bs, block_magic(bs), nsub, pre_s, suf_s,

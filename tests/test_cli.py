@@ -87,3 +87,25 @@ def test_gpu_tie_arrays_and_nb(gpu):
     for kind in ("ties16", "desc", "const"):
         r = run(300_007, 50_000, -2, 105, "--nb", 97, "--reps", 1, "--seed", 4, "--array", kind, "--check")
         assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
+
+
+def test_engine_options_parse_for_cpu_algs():
+    r = run(3000, 100, -1, 0, "--edges", "--tune", "--nt", 1, "--reps", 1, "--seed", 1)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", [5, 105])
+def test_gpu_edges_and_autotune(alg, tmp_path, gpu):
+    r = run(1 << 20, 1 << 16, -2, alg, "--edges", "--bs", 1000, "--reps", 2, "--seed", 5, "--check")
+    assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
+    assert "edge tables" in r.stdout
+    csv, cache = tmp_path / "t.csv", tmp_path / "picks.txt"
+    args = (1 << 20, 1 << 16, -1, alg, f"--tune={cache}", "--reps", 2, "--seed", 5, "--check", f"--save-time={csv}")
+    r = run(*args)
+    assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
+    m = re.search(r"\[bs=(\d+) edges=(\d)", r.stdout)
+    assert m and "; timed]" in r.stdout
+    assert csv.read_text().strip().split(",")[4] == m.group(1)  # the CSV row carries the picked bs
+    r = run(*args)
+    assert r.returncode == 0 and "; cached]" in r.stdout and "pass (0 mismatches)" in r.stdout
