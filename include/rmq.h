@@ -177,9 +177,12 @@ int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t 
  *        (kept alive by the caller) after checking that its block table fits
  *        in free HBM beside the request budget; returns its id.
  *      - rmq_server_submit queues a request: q host query pairs for one array
- *        and host output buffers (pinned memory gives asynchronous copies);
- *        it returns at once with a ticket.  The buffers must stay valid until
- *        rmq_server_wait returns for that ticket.
+ *        and host output buffers; it returns at once with a ticket.  The
+ *        buffers must stay valid, and the queries unchanged, until
+ *        rmq_server_wait returns for that ticket.  Pinned buffers
+ *        (hipHostMalloc, torch's pin_memory) are read and written by the
+ *        kernel directly over PCIe: no staging copies and no device buffer.
+ *        Pageable buffers are staged through device memory from the budget.
  *      - A scheduler thread admits pending requests FIFO while their device
  *        buffers fit in the memory budget and coalesces them, up to
  *        max_batch_queries, into one plan launch on a free slot (stream):
@@ -195,13 +198,14 @@ int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t 
  *      Answers are the same as rmq_query's for the same array and queries,
  *      whatever else shares the launch. ---- */
 typedef struct rmq_server rmq_server_t;
+#define RMQ_SERVER_STAGE_COPIES 1u
 typedef struct rmq_server_config {
     int32_t slots;             /* launches in flight at once (0: 3)                    */
     int32_t max_arrays;        /* resident arrays (0: 1024)                            */
     int64_t mem_budget_bytes;  /* cap on request device memory in flight (0: from HBM) */
     int64_t max_batch_queries; /* queries coalesced into one launch (0: 2^24)          */
     int32_t item_queries;      /* queries per work item (0: sized per launch, 64..4096) */
-    int32_t reserved;
+    int32_t flags;             /* RMQ_SERVER_STAGE_COPIES: stage pinned buffers too     */
     rmq_query_opts_t opts;     /* applied to every array (zeros: defaults)             */
 } rmq_server_config_t;
 typedef struct rmq_server_stats {

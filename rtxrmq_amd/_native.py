@@ -24,6 +24,7 @@ RMQ_ERR_BAD_QUERY = -5
 
 GEN_KINDS = {"uniform": 0, "ties16": 1, "desc": 2, "const": 3}
 BUILD_EDGES = 1  # RMQ_BUILD_EDGES
+SERVER_STAGE_COPIES = 1  # RMQ_SERVER_STAGE_COPIES
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -58,7 +59,7 @@ class TuneResult(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     """rmq_server_config_t"""
     _fields_ = [("slots", _i32), ("max_arrays", _i32), ("mem_budget_bytes", _i64), ("max_batch_queries", _i64),
-                ("item_queries", _i32), ("reserved", _i32), ("opts", QueryOpts)]
+                ("item_queries", _i32), ("flags", _i32), ("opts", QueryOpts)]
 
 
 class ServerStats(ctypes.Structure):

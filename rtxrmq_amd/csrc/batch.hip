@@ -320,7 +320,13 @@ struct Request {
     int64_t q = 0;
     int32_t *h_idx = nullptr;
     float *h_val = nullptr;
-    int64_t bytes = 0;  // device bytes while in flight
+    // Device-visible addresses of pinned host buffers: the kernel reads the
+    // queries and writes the answers over PCIe, with no staging copies.
+    bool zero_copy = false;
+    const int2 *z_Q = nullptr;
+    int32_t *z_idx = nullptr;
+    float *z_val = nullptr;
+    int64_t bytes = 0;  // device bytes while in flight (0 when zero_copy)
     double t_submit = 0, t_done = 0;
     int status = RMQ_OK;
     bool done = false;
@@ -347,7 +353,24 @@ int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
 // Device bytes a request holds while in flight: its queries and outputs,
 // each rounded to 256 bytes (the carving launch_group does).
 int64_t request_bytes(const Request &r) {
+    if (r.zero_copy) return 0;
     return round256(r.q * 8) + (r.h_idx ? round256(r.q * 4) : 0) + (r.h_val ? round256(r.q * 4) : 0);
+}
+
+// Device-visible address of a pinned host buffer (hipHostMalloc, torch's
+// pin_memory), or nullptr for pageable memory, which must be staged.
+template <typename T>
+T *device_view(T *p) {
+    if (!p) return nullptr;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unknown pointer leaves an error behind
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer || !a.hostPointer) return nullptr;
+    char *d = static_cast<char *>(a.devicePointer) +
+              (reinterpret_cast<const char *>(p) - static_cast<const char *>(a.hostPointer));
+    return reinterpret_cast<T *>(d);
 }
 
 }  // namespace
@@ -402,29 +425,35 @@ int launch_group(rmq_server *sv, Slot &sl, std::vector<Request *> &group) {
         sl.items_cap = cap;
     }
     int rc = hip_rc(hipEventRecord(sl.start_ev, sl.stream));
-    if (rc == RMQ_OK) rc = hip_rc(hipMallocAsync(&sl.d_buf, (size_t)total, sl.stream));
+    // staged requests (pageable host buffers) share one stream-ordered buffer
+    if (rc == RMQ_OK && total > 0) rc = hip_rc(hipMallocAsync(&sl.d_buf, (size_t)total, sl.stream));
     if (rc != RMQ_OK) return rc;
     char *p = static_cast<char *>(sl.d_buf);
     size_t k = 0;
     for (Request *r : group) {
-        int32_t *d_Q = reinterpret_cast<int32_t *>(p);
-        p += round256(r->q * 8);
-        int32_t *d_idx = nullptr;
-        float *d_val = nullptr;
-        if (r->h_idx) {
-            d_idx = reinterpret_cast<int32_t *>(p);
-            p += round256(r->q * 4);
+        const int2 *d_Q = r->z_Q;
+        int32_t *d_idx = r->z_idx;
+        float *d_val = r->z_val;
+        if (!r->zero_copy) {
+            d_Q = reinterpret_cast<const int2 *>(p);
+            p += round256(r->q * 8);
+            d_idx = nullptr;
+            d_val = nullptr;
+            if (r->h_idx) {
+                d_idx = reinterpret_cast<int32_t *>(p);
+                p += round256(r->q * 4);
+            }
+            if (r->h_val) {
+                d_val = reinterpret_cast<float *>(p);
+                p += round256(r->q * 4);
+            }
+            if (rc == RMQ_OK)
+                rc = hip_rc(hipMemcpyAsync(const_cast<int2 *>(d_Q), r->h_Q, (size_t)r->q * 8, hipMemcpyHostToDevice,
+                                           sl.stream));
         }
-        if (r->h_val) {
-            d_val = reinterpret_cast<float *>(p);
-            p += round256(r->q * 4);
-        }
-        if (rc == RMQ_OK)
-            rc = hip_rc(hipMemcpyAsync(d_Q, r->h_Q, (size_t)r->q * 8, hipMemcpyHostToDevice, sl.stream));
         for (int64_t off = 0; off < r->q; off += iq) {
-            sl.h_items[k++] = WorkItem{reinterpret_cast<const int2 *>(d_Q) + off, d_val ? d_val + off : nullptr,
-                                       d_idx ? d_idx + off : nullptr, (int32_t)std::min<int64_t>(iq, r->q - off),
-                                       r->array};
+            sl.h_items[k++] = WorkItem{d_Q + off, d_val ? d_val + off : nullptr, d_idx ? d_idx + off : nullptr,
+                                       (int32_t)std::min<int64_t>(iq, r->q - off), r->array};
         }
     }
     if (rc == RMQ_OK)
@@ -438,9 +467,10 @@ int launch_group(rmq_server *sv, Slot &sl, std::vector<Request *> &group) {
                            sl.d_head, sv->cfg.opts.short_max);
         rc = hip_rc(hipGetLastError());
     }
-    // copy the answers back, request by request
+    // copy the staged answers back, request by request
     p = static_cast<char *>(sl.d_buf);
     for (Request *r : group) {
+        if (r->zero_copy) continue;
         p += round256(r->q * 8);
         if (r->h_idx) {
             if (rc == RMQ_OK)
@@ -453,7 +483,8 @@ int launch_group(rmq_server *sv, Slot &sl, std::vector<Request *> &group) {
             p += round256(r->q * 4);
         }
     }
-    const int rc_free = hip_rc(hipFreeAsync(sl.d_buf, sl.stream));
+    int rc_free = RMQ_OK;
+    if (sl.d_buf) rc_free = hip_rc(hipFreeAsync(sl.d_buf, sl.stream));
     sl.d_buf = nullptr;
     const int rc_ev = hip_rc(hipEventRecord(sl.done_ev, sl.stream));
     return rc != RMQ_OK ? rc : (rc_free != RMQ_OK ? rc_free : rc_ev);
@@ -605,7 +636,8 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
     *out = nullptr;
     rmq_server_config_t c{};
     if (cfg) c = *cfg;
-    if (c.slots < 0 || c.max_arrays < 0 || c.mem_budget_bytes < 0 || c.max_batch_queries < 0 || c.item_queries < 0)
+    if (c.slots < 0 || c.max_arrays < 0 || c.mem_budget_bytes < 0 || c.max_batch_queries < 0 || c.item_queries < 0 ||
+        (c.flags & ~(int32_t)RMQ_SERVER_STAGE_COPIES))
         return RMQ_ERR_INVALID_ARG;
     if (c.slots == 0) c.slots = 3;
     if (c.max_arrays == 0) c.max_arrays = 1024;
@@ -716,6 +748,12 @@ RMQ_API int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t 
     r.q = q;
     r.h_idx = h_idx;
     r.h_val = h_val;
+    if (!(sv->cfg.flags & RMQ_SERVER_STAGE_COPIES)) {
+        r.z_Q = device_view(reinterpret_cast<const int2 *>(h_Q));
+        r.z_idx = device_view(h_idx);
+        r.z_val = device_view(h_val);
+        r.zero_copy = r.z_Q && (!h_idx || r.z_idx) && (!h_val || r.z_val);
+    }
     r.bytes = request_bytes(r);
     r.t_submit = now_ms();
     if (sv->first_submit < 0) sv->first_submit = r.t_submit;

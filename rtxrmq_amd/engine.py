@@ -303,12 +303,14 @@ class Server:
 
     add_array keeps the array tensor referenced; submit keeps the request's
     host buffers referenced until wait returns.  Host buffers may be numpy
-    arrays or (for asynchronous copies) pinned torch tensors.
+    arrays (staged through device memory) or pinned torch tensors, which the
+    kernel reads and writes directly unless stage_copies is set.
     """
 
     def __init__(self, slots: int = 0, mem_budget_bytes: int = 0, max_batch_queries: int = 0,
-                 item_queries: int = 0, opts: N.QueryOpts | None = None):
-        cfg = N.ServerConfig(slots, 0, mem_budget_bytes, max_batch_queries, item_queries, 0,
+                 item_queries: int = 0, opts: N.QueryOpts | None = None, stage_copies: bool = False):
+        cfg = N.ServerConfig(slots, 0, mem_budget_bytes, max_batch_queries, item_queries,
+                             N.SERVER_STAGE_COPIES if stage_copies else 0,
                              opts if opts is not None else N.QueryOpts())
         h = ctypes.c_void_p()
         N.check(N.lib().rmq_server_create(ctypes.byref(cfg), ctypes.byref(h)), "rmq_server_create")
@@ -325,23 +327,27 @@ class Server:
         return aid.value
 
     @staticmethod
-    def _host_ptr(x):
+    def _host_ptr(x, kind: str, min_count: int):
+        """(object to keep alive, address) of a contiguous host buffer of at
+        least min_count elements of `kind` ("int32" or "float32")."""
         if x is None:
             return None, None
         if isinstance(x, np.ndarray):
-            if not x.flags["C_CONTIGUOUS"]:
-                raise TypeError("host buffers must be contiguous")
+            if not x.flags["C_CONTIGUOUS"] or x.dtype != np.dtype(kind) or x.size < min_count:
+                raise TypeError(f"host buffer must be a contiguous {kind} array of at least {min_count} entries")
             return x, x.ctypes.data
-        if x.device.type != "cpu" or not x.is_contiguous():
-            raise TypeError("host buffers must be contiguous CPU tensors")
+        torch = _torch()
+        if (x.device.type != "cpu" or not x.is_contiguous() or x.dtype != getattr(torch, kind)
+                or x.numel() < min_count):
+            raise TypeError(f"host buffer must be a contiguous CPU {kind} tensor of at least {min_count} entries")
         return x, x.data_ptr()
 
     def submit(self, array_id: int, Q, out_idx=None, out_val=None) -> int:
         """Q: (q, 2) int32 host queries; out_idx / out_val: host outputs of q entries."""
         q = (Q.size if isinstance(Q, np.ndarray) else Q.numel()) // 2
-        keep_q, pq = self._host_ptr(Q)
-        keep_i, pi = self._host_ptr(out_idx)
-        keep_v, pv = self._host_ptr(out_val)
+        keep_q, pq = self._host_ptr(Q, "int32", 2 * q)
+        keep_i, pi = self._host_ptr(out_idx, "int32", q)
+        keep_v, pv = self._host_ptr(out_val, "float32", q)
         tk = ctypes.c_uint64()
         N.check(N.lib().rmq_server_submit(self._h, array_id, pq, q, pi, pv, ctypes.byref(tk)), "rmq_server_submit")
         self._live[tk.value] = (keep_q, keep_i, keep_v)

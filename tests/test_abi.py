@@ -62,6 +62,8 @@ def test_argument_validation_before_device_use():
     cfg = N.ServerConfig(-1, 0, 0, 0, 0, 0, R.QueryOpts())
     srv = ctypes.c_void_p()
     assert lib.rmq_server_create(ctypes.byref(cfg), ctypes.byref(srv)) == R.RMQ_ERR_INVALID_ARG
+    cfg = N.ServerConfig(0, 0, 0, 0, 0, 2, R.QueryOpts())  # unknown server flag
+    assert lib.rmq_server_create(ctypes.byref(cfg), ctypes.byref(srv)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_server_submit(None, 0, fake, 4, fake, None, None) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_server_wait(None, 1, None) == R.RMQ_ERR_INVALID_ARG
     lib.rmq_free(None)  # no-op

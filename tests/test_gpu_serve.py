@@ -118,3 +118,42 @@ def test_derived_budget_yields_to_resident_arrays(arrays, gpu):
         assert sv.wait(sv.submit(aid, Q, out_idx=out))[0] == R.RMQ_OK
     assert 0 < b1 < b0
     np.testing.assert_array_equal(out, O.rmq_brute(arrays[2][0], Q)[1])
+
+
+@pytest.mark.parametrize("stage", [False, True])
+def test_pinned_buffers_reused_with_new_contents(arrays, stage, gpu):
+    """Pinned buffers are read and written by the kernel in place (unless
+    staging is forced); refilling them between requests must be seen, and an
+    interior slice must be written exactly, nothing around it."""
+    A, Ad = arrays[2]
+    n, q = A.size, 30000
+    with R.Server(slots=2, stage_copies=stage) as sv:
+        aid = sv.add_array(Ad, 256, R.BUILD_EDGES)
+        Qp = torch.empty((q, 2), dtype=torch.int32, pin_memory=True)
+        big_i = torch.empty(q + 6000, dtype=torch.int32, pin_memory=True)
+        big_v = torch.empty(q + 6000, dtype=torch.float32, pin_memory=True)
+        out_i, out_v = big_i[5000:5000 + q], big_v[5000:5000 + q]
+        for k, lr in enumerate((-1, -7, -8, 5)):
+            Q = O.gen_queries(q, n, lr, 500 + k)
+            Qp.numpy()[:] = Q
+            big_i.fill_(-9)
+            big_v.fill_(-2.0)
+            rc, _ = sv.wait(sv.submit(aid, Qp, out_idx=out_i, out_val=out_v))
+            assert rc == R.RMQ_OK
+            ev, ei = O.rmq_brute(A, Q)
+            np.testing.assert_array_equal(out_i.numpy(), ei, err_msg=f"batch {k}")
+            np.testing.assert_array_equal(out_v.numpy().view(np.int32), ev.view(np.int32), err_msg=f"batch {k}")
+            assert np.all(big_i.numpy()[:5000] == -9) and np.all(big_i.numpy()[5000 + q:] == -9)
+            assert np.all(big_v.numpy()[:5000] == -2.0) and np.all(big_v.numpy()[5000 + q:] == -2.0)
+        # pinned queries with pageable outputs: staged
+        Q = O.gen_queries(q, n, -1, 600)
+        Qp.numpy()[:] = Q
+        host_out = np.empty(q, np.int32)
+        assert sv.wait(sv.submit(aid, Qp, out_idx=host_out))[0] == R.RMQ_OK
+        np.testing.assert_array_equal(host_out, O.rmq_brute(A, Q)[1])
+        st = sv.stats()
+        with pytest.raises(TypeError):
+            sv.submit(aid, Qp, out_idx=torch.empty(q, dtype=torch.int64, pin_memory=True))
+        with pytest.raises(TypeError):
+            sv.submit(aid, Qp, out_idx=np.empty(q - 1, np.int32))
+    assert st["failed"] == 0 and st["peak_inflight_bytes"] > 0  # the staged request used device memory
