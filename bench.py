@@ -326,7 +326,8 @@ def run_batched(R, torch, dev, args) -> dict:
                         "rmq_gen_queries")
     torch.cuda.synchronize()
     big = int(np.argmax(ns))
-    pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096])
+    pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096],
+                  cache_path=args.tune_cache or None, refresh=args.retune)
     opts, bs, flags = pick["opts"], pick["bs"], pick["flags"]
     log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} edges={bool(flags)} "
         f"short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
