@@ -52,31 +52,107 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(Q: np.ndarray, bs: int, short_max: int, out_bytes: int = 8,
-                      edges: bool = False) -> tuple[float, float, int, int]:
-    """(bytes in the long kernel, bytes in the short kernel, n_long, n_short) per pass.
+class BlockLevel:
+    """Copy of an index's block level for the byte model: (min, leftmost
+    argmin) per block and a sparse table over them, as torch tensors on the
+    array's device, to replay which edge parts each query needs."""
 
-    Scan path: B = 8 + 4*S + T + out.  With edge tables a long range (more than
-    max(short_max, 32) elements) scans nothing: B = 8 + 16 (suffix and prefix
-    pairs) + T + out, T = 16 when whole blocks (or, inside one block, whole
-    32-element sub-blocks) lie between the two edges."""
-    l = Q[:, 0].astype(np.int64)
-    r = Q[:, 1].astype(np.int64)
-    ln = r - l + 1
-    lB, rB = l // bs, r // bs
-    if edges:
-        short_max = max(short_max, 32)
-        jl, jr = (l - lB * bs) // 32, (r - lB * bs) // 32
-        mid = np.where(lB != rB, rB - lB > 1, jr - jl > 1)
-        B = 8 + 16 + np.where(mid, 16, 0) + out_bytes
-    else:
+    def __init__(self, A, bs: int):
+        import torch
+        n, dev = A.numel(), A.device
+        nb = -(-n // bs)
+        pad = torch.full((nb * bs,), float("inf"), dtype=torch.float32, device=dev)
+        pad[:n] = A
+        rows = pad.view(nb, bs)
+        v = rows.min(dim=1).values
+        pos = torch.arange(bs, device=dev).expand(nb, bs)
+        j = torch.where(rows == v[:, None], pos, bs).min(dim=1).values  # leftmost; all +inf: 0
+        i = torch.arange(nb, device=dev, dtype=torch.int64) * bs + j
+        del pad, rows, pos
+        self.bs, self.nb = bs, nb
+        levels = max(1, nb.bit_length())
+        self.V = torch.full((levels, nb), float("inf"), dtype=torch.float32, device=dev)
+        self.I = torch.full((levels, nb), 2 ** 31 - 1, dtype=torch.int64, device=dev)
+        self.V[0], self.I[0] = v, i
+        for k in range(1, levels):
+            h, cnt = 1 << (k - 1), nb - (1 << k) + 1
+            xv, xi = self.V[k - 1, :cnt], self.I[k - 1, :cnt]
+            yv, yi = self.V[k - 1, h:h + cnt], self.I[k - 1, h:h + cnt]
+            t = yv < xv  # the left window wins ties
+            self.V[k, :cnt] = torch.where(t, yv, xv)
+            self.I[k, :cnt] = torch.where(t, yi, xi)
+
+    def range_min(self, a, c):
+        """Leftmost minimum of blocks [a, a+c) for c >= 1, elementwise."""
+        import torch
+        lev = (torch.frexp(c.double())[1] - 1).long()  # floor(log2 c), exact for c < 2^53
+        b = a + c - (1 << lev)
+        xv, xi, yv, yi = self.V[lev, a], self.I[lev, a], self.V[lev, b], self.I[lev, b]
+        t = yv < xv
+        return torch.where(t, yv, xv), torch.where(t, yi, xi)
+
+
+def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False) -> dict:
+    """Per pass: bytes in the long and the short kernel, their query counts,
+    the fraction of cross-block edge parts computed, and the random gathers
+    of per-element edge pairs (with edge tables).  Q: (q, 2) int32 tensor on
+    the level's device.
+
+    Short class (<= short_max elements, >= 32 with edge tables) and ranges
+    inside one block without edge tables: B = 8 + 4*len + out.  Inside one
+    block with edge tables: B = 8 + 16 (suffix and prefix pairs) + T + out.
+    Across blocks: B = 8 + T + 16 (the two edge blocks' (min, argmin)) + each
+    edge part the query needs + out, where an edge part is needed unless its
+    block's leftmost argmin lies inside it (then that entry is exact) or its
+    block minimum cannot win (left: > the best so far; right: >= it); a needed
+    part costs 4 B per element scanned, or 8 B (one edge pair) with edge
+    tables.  T = 16 when whole blocks (or sub-blocks) lie between."""
+    import torch
+    bs = lvl.bs
+    long_b = short_b = 0.0
+    n_long = n_short = parts = needed = gathers = 0
+    sm = max(short_max, 32) if edges else short_max
+    for c0 in range(0, Q.shape[0], 1 << 23):
+        l = Q[c0:c0 + (1 << 23), 0].long()
+        r = Q[c0:c0 + (1 << 23), 1].long()
+        ln = r - l + 1
+        is_long = ln > sm
+        short_b += float((8 + 4 * ln[~is_long] + out_bytes).sum())
+        n_short += int((~is_long).sum())
+        l, r, ln = l[is_long], r[is_long], ln[is_long]
+        n_long += int(l.numel())
+        lB, rB = l // bs, r // bs
         same = lB == rB
-        S = np.where(same, ln, (bs - l % bs) + (r % bs + 1))
-        T = np.where(rB - lB > 1, 16, 0)
-        B = 8 + 4 * S + T + out_bytes
-    is_long = ln > short_max
-    short_B = 8 + 4 * ln + out_bytes
-    return float(B[is_long].sum()), float(short_B[~is_long].sum()), int(is_long.sum()), int((~is_long).sum())
+        if edges:
+            jl, jr = (l[same] - lB[same] * bs) // 32, (r[same] - lB[same] * bs) // 32
+            long_b += float((8 + 16 + 16 * (jr - jl > 1) + out_bytes).sum())
+            gathers += 2 * int(same.sum())
+        else:
+            long_b += float((8 + 4 * ln[same] + out_bytes).sum())
+        x = ~same
+        l, r, lB, rB = l[x], r[x], lB[x], rB[x]
+        mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
+        mi = torch.full_like(l, 2 ** 31 - 1)
+        mid = rB - lB > 1
+        if bool(mid.any()):
+            mv[mid], mi[mid] = lvl.range_min(lB[mid] + 1, rB[mid] - lB[mid] - 1)
+        ev, ei = lvl.V[0, lB], lvl.I[0, lB]
+        exact_l = ei >= l
+        need_l = ~exact_l & (ev <= mv)
+        cv = torch.where(exact_l & ((ev < mv) | ((ev == mv) & (ei < mi))), ev, mv)
+        fv, fi = lvl.V[0, rB], lvl.I[0, rB]
+        need_r = (fi > r) & (fv < cv)
+        parts += 2 * int(l.numel())
+        n_need = int(need_l.sum() + need_r.sum())
+        needed += n_need
+        if edges:
+            edge_b = 8 * need_l.long() + 8 * need_r.long()
+            gathers += n_need
+        else:
+            edge_b = 4 * need_l.long() * ((lB + 1) * bs - l) + 4 * need_r.long() * (r - rB * bs + 1)
+        long_b += float((8 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
+    return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
+            "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers}
 
 
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
@@ -244,13 +320,15 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
     return steps_out, time.perf_counter() - t_start
 
 
-def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -> dict:
-    """RMQ/s, per-distribution figures and the roofline of the dominant kernel."""
+def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None) -> dict:
+    """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
+    levels: host BlockLevel per block size."""
     per_dist = {}
-    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches]
+    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, random gathers]
     for lr in DISTS:
-        lb, sb, nl, ns = algorithmic_bytes(Q_host[lr], bs_for[lr], opts_for[lr].short_max,
-                                           edges=bool(edges_for and edges_for[lr]))
+        ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max,
+                               edges=bool(edges_for and edges_for[lr]))
+        lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]
         sms = [s[lr]["short_ms"] for s in steps_out]
@@ -263,6 +341,7 @@ def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -
             "long_kernel_ms": float(np.mean(lms)),
             "short_kernel_ms": float(np.mean(sms)),
             "n_long": nl,
+            "edge_parts_computed": ab["edge_parts_computed"],
             "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
             "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
         }
@@ -270,7 +349,7 @@ def summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for, edges_for=None) -
                     else f"k_query_tiles<{opts_for[lr].long_lanes}, true>")
         short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
         per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
-        gathers = 2 * nl if edges_for and edges_for[lr] else 0  # suffix pair at l, prefix pair at r
+        gathers = ab["gathers"]  # per-element edge pairs read (edge tables only)
         for sym, b, t, g in ((long_sym, lb, lms, gathers), (short_sym, sb, sms, 0)):
             acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0])
             acc[0] += b * len(steps_out)
@@ -578,7 +657,8 @@ def main() -> int:
     fixed_steps, fixed_elapsed = timed_run(R, torch, {lr: fixed_index for lr in DISTS},
                                            {lr: fixed_opts for lr in DISTS}, Q_dev, out_v, out_i,
                                            args.fixed_steps, 1, f"bs={args.bs}")
-    fixed = summarise(Q_host, q, fixed_steps, fixed_elapsed, {lr: args.bs for lr in DISTS},
+    levels = {args.bs: BlockLevel(A, args.bs)}
+    fixed = summarise(Q_dev, q, fixed_steps, fixed_elapsed, levels, {lr: args.bs for lr in DISTS},
                       {lr: fixed_opts for lr in DISTS})
     fixed_index.close()
 
@@ -606,7 +686,9 @@ def main() -> int:
     bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
     steps_out, elapsed = timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, args.steps, args.warmup,
                                    "tuned")
-    tuned = summarise(Q_host, q, steps_out, elapsed, bs_for, opts_for,
+    for b in set(bs_for.values()) - set(levels):
+        levels[b] = BlockLevel(A, b)
+    tuned = summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
                       {lr: picks[lr]["flags"] for lr in DISTS})
     log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
@@ -629,9 +711,9 @@ def main() -> int:
     batched = None if args.no_batched else run_batched(R, torch, dev, args)
     served = None if args.no_served else run_served(R, torch, dev, args)
     cpu = None
+    del levels
     if not args.no_cpu_baseline:
-        A_host = A.cpu().numpy()
-        cpu = cpu_baseline(A_host, Q_host, args.cpu_seconds)
+        cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds)
 
     line = {
         "metric": "RMQ/s at n=1e8, q=2^26 over the three length distributions (uniform, lognormal 2^14, "

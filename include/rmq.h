@@ -133,7 +133,7 @@ typedef struct rmq_tune_result {
 /* ---- autotuner (no counterpart in the reference, whose sweeps are shell
  *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
  *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^3 ..
- *      2^16), with and without edge tables (RMQ_BUILD_EDGES), then the
+ *      2^18), with and without edge tables (RMQ_BUILD_EDGES), then the
  *      short-range threshold and lanes per query at the best block size, and
  *      returns the fastest configuration (build it with rmq_build(..., bs,
  *      build_flags, ...)).  Results are cached

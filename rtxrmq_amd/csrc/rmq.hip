@@ -814,7 +814,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     if (n_bs > 0) {
         cands.assign(bs_list, bs_list + n_bs);
     } else {
-        for (int b = 3; b <= 16; ++b) cands.push_back(1 << b);
+        for (int b = 3; b <= 18; ++b) cands.push_back(1 << b);
     }
     auto base_opts = [](int32_t bs) {
         rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, bs <= 2048 ? 16 : kWave};

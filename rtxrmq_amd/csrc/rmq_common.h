@@ -202,8 +202,17 @@ __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int 
                 mv = m.v;
                 mi = m.i;
             }
-            scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);
-            scan_range<G, U>(A, rB * bs, r, gl, bv, bi);
+            // Edge blocks, from their (min, leftmost argmin) in table row 0:
+            // exact when that argmin lies inside the edge part; skipped when
+            // the block minimum, a lower bound for the part, cannot win; only
+            // otherwise scanned.  (mv, mi) is known to every lane of the group,
+            // so the group takes the same branch.
+            const Pair eb = st[lB];
+            if (eb.i >= l) take_if_better(eb.v, eb.i, mv, mi);
+            else if (eb.v <= mv) scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);  // earlier: wins ties
+            const Pair er = st[rB];
+            if (er.i <= r) take_if_better(er.v, er.i, mv, mi);
+            else if (er.v < mv) scan_range<G, U>(A, rB * bs, r, gl, bv, bi);  // later: must be smaller
         }
     }
 }
@@ -246,12 +255,26 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Pair *__
     const int bl = block_of(l, magic), br = block_of(r, magic);
     Pair best, right;
     if (bl != br) {
-        best = e.suf_b[l];
-        right = e.pre_b[r];
-        if (br - bl > 1) {
-            const Pair mid = table_min(st, nb, bl + 1, br - bl - 1);
-            if (mid.v < best.v) best = mid;
+        // Whole blocks between, then each edge from its block's (min,
+        // leftmost argmin) in table row 0: exact when the argmin lies in the
+        // edge part, skipped when the block minimum cannot win, and only
+        // otherwise read from the per-element edge table.
+        best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
+        const Pair el = st[bl];
+        if (el.i >= l) {
+            if (el.v <= best.v) best = el;  // earlier indices win ties
+        } else if (el.v <= best.v) {
+            const Pair x = e.suf_b[l];
+            if (x.v <= best.v) best = x;
         }
+        const Pair er = st[br];
+        if (er.i <= r) {
+            if (er.v < best.v) best = er;  // later indices must be smaller
+        } else if (er.v < best.v) {
+            const Pair x = e.pre_b[r];
+            if (x.v < best.v) best = x;
+        }
+        return best;
     } else {
         const int base = bl * bs;
         const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
