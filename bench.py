@@ -92,21 +92,26 @@ class BlockLevel:
         return torch.where(t, yv, xv), torch.where(t, yi, xi)
 
 
-def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False) -> dict:
+def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
+                      sub: BlockLevel | None = None) -> dict:
     """Per pass: bytes in the long and the short kernel, their query counts,
     the fraction of cross-block edge parts computed, and the random gathers
     of per-element edge pairs (with edge tables).  Q: (q, 2) int32 tensor on
     the level's device.
 
     Short class (<= short_max elements, >= 32 with edge tables) and ranges
-    inside one block without edge tables: B = 8 + 4*len + out.  Inside one
-    block with edge tables: B = 8 + 16 (suffix and prefix pairs) + T + out.
-    Across blocks: B = 8 + T + 16 (the two edge blocks' (min, argmin)) + each
-    edge part the query needs + out, where an edge part is needed unless its
+    inside one block without edge tables: B = 8 + 4*len + out.  Every other
+    range first reads the minimum of the blocks it touches (of the 32-element
+    sub-blocks it touches, inside one block; two table entries, 16 B): when
+    its argmin lies in [l, r] that is the answer, B = 8 + 16 + out.
+    Otherwise, inside one block (edge tables): + 16 (suffix and prefix pairs)
+    + T.  Across blocks: + T + 16 (the two edge blocks' (min, argmin)) + each
+    edge part the query needs, where an edge part is needed unless its
     block's leftmost argmin lies inside it (then that entry is exact) or its
     block minimum cannot win (left: > the best so far; right: >= it); a needed
     part costs 4 B per element scanned, or 8 B (one edge pair) with edge
-    tables.  T = 16 when whole blocks (or sub-blocks) lie between."""
+    tables.  T = 16 when whole blocks (or sub-blocks) lie between.  sub: the
+    32-element level (block sizes that are multiples of 32), with edge tables."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
@@ -124,12 +129,21 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         lB, rB = l // bs, r // bs
         same = lB == rB
         if edges:
-            jl, jr = (l[same] - lB[same] * bs) // 32, (r[same] - lB[same] * bs) // 32
-            long_b += float((8 + 16 + 16 * (jr - jl > 1) + out_bytes).sum())
-            gathers += 2 * int(same.sum())
+            ls, rs = l[same], r[same]
+            gl, gr = ls // 32, rs // 32  # the sub-block grid restarts at block starts: bs % 32 == 0
+            sup_i = sub.range_min(gl, gr - gl + 1)[1]
+            slow = ~((sup_i >= ls) & (sup_i <= rs))
+            long_b += float((8 + 16 + slow.long() * (16 + 16 * (gr - gl > 1).long()) + out_bytes).sum())
+            gathers += 2 * int(slow.sum())
         else:
             long_b += float((8 + 4 * ln[same] + out_bytes).sum())
         x = ~same
+        l, r, lB, rB = l[x], r[x], lB[x], rB[x]
+        sup_i = lvl.range_min(lB, rB - lB + 1)[1]
+        fast = (sup_i >= l) & (sup_i <= r)
+        long_b += float((8 + 16 + out_bytes) * fast.sum())
+        parts += 2 * int(fast.sum())
+        x = ~fast
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
         mi = torch.full_like(l, 2 ** 31 - 1)
@@ -150,7 +164,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             gathers += n_need
         else:
             edge_b = 4 * need_l.long() * ((lB + 1) * bs - l) + 4 * need_r.long() * (r - rB * bs + 1)
-        long_b += float((8 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
+        long_b += float((8 + 16 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
             "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers}
 
@@ -326,8 +340,11 @@ def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=
     per_dist = {}
     kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, random gathers]
     for lr in DISTS:
-        ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max,
-                               edges=bool(edges_for and edges_for[lr]))
+        e = bool(edges_for and edges_for[lr])
+        if e and bs_for[lr] % 32:
+            raise ValueError("the byte model needs block sizes that are multiples of 32 with edge tables")
+        ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
+                               sub=levels.get(32) if e else None)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]
@@ -686,7 +703,7 @@ def main() -> int:
     bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
     steps_out, elapsed = timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, args.steps, args.warmup,
                                    "tuned")
-    for b in set(bs_for.values()) - set(levels):
+    for b in (set(bs_for.values()) | ({32} if any(p["flags"] for p in picks.values()) else set())) - set(levels):
         levels[b] = BlockLevel(A, b)
     tuned = summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
                       {lr: picks[lr]["flags"] for lr in DISTS})

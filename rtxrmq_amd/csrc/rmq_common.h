@@ -181,6 +181,16 @@ __device__ __forceinline__ int block_of(int e, uint64_t magic) {
     return magic ? (int)__umul64hi((uint64_t)e, magic) : e;
 }
 
+// Leftmost minimum of the c >= 1 entries [a, a+c) of a sparse table whose
+// rows hold `stride` entries; the left window wins ties.
+__device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t stride, int64_t a, int c) {
+    const int lev = 31 - __clz(c);
+    const Pair *row = st + (int64_t)lev * stride;
+    const Pair x = row[a];
+    const Pair y = row[a + c - (1 << lev)];
+    return (y.v < x.v) ? y : x;
+}
+
 template <int G, int U, bool LONG>
 __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, uint64_t magic, int nb,
                                                const Pair *__restrict__ st, int l, int r, int gl, float &bv,
@@ -192,13 +202,17 @@ __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int 
         if (lB == rB) {
             scan_range<G, U>(A, l, r, gl, bv, bi);
         } else {
+            // The minimum of the whole blocks lB..rB covers the range; when
+            // its leftmost argmin lies inside [l, r] it is the answer (every
+            // element before it in those blocks is larger).
+            const Pair sup = table_min(st, nb, lB, rB - lB + 1);
+            if (sup.i >= l && sup.i <= r) {
+                mv = sup.v;
+                mi = sup.i;
+                return;
+            }
             if (rB - lB > 1) {
-                const int a = lB + 1, c = rB - lB - 1;
-                const int lev = 31 - __clz(c);
-                const Pair *row = st + (int64_t)lev * nb;
-                const Pair x = row[a];
-                const Pair y = row[a + c - (1 << lev)];
-                const Pair m = (y.v < x.v) ? y : x;
+                const Pair m = table_min(st, nb, lB + 1, rB - lB - 1);
                 mv = m.v;
                 mi = m.i;
             }
@@ -234,15 +248,6 @@ struct EdgeTables {
     int32_t levels_s;
 };
 
-// Leftmost minimum of the c >= 1 entries [a, a+c) of a sparse table whose
-// rows hold `stride` entries; the left window wins ties.
-__device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t stride, int64_t a, int c) {
-    const int lev = 31 - __clz(c);
-    const Pair *row = st + (int64_t)lev * stride;
-    const Pair x = row[a];
-    const Pair y = row[a + c - (1 << lev)];
-    return (y.v < x.v) ? y : x;
-}
 
 // O(1) answer of a range of more than 32 elements from the edge tables:
 // blocks differ -> suffix of the left block, whole blocks between (block
@@ -255,10 +260,13 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Pair *__
     const int bl = block_of(l, magic), br = block_of(r, magic);
     Pair best, right;
     if (bl != br) {
-        // Whole blocks between, then each edge from its block's (min,
-        // leftmost argmin) in table row 0: exact when the argmin lies in the
-        // edge part, skipped when the block minimum cannot win, and only
-        // otherwise read from the per-element edge table.
+        // The minimum of blocks bl..br, when its leftmost argmin lies in
+        // [l, r], is the answer.  Otherwise: whole blocks between, then each
+        // edge from its block's (min, leftmost argmin) in table row 0: exact
+        // when the argmin lies in the edge part, skipped when the block
+        // minimum cannot win, and only otherwise read from the edge table.
+        const Pair sup = table_min(st, nb, bl, br - bl + 1);
+        if (sup.i >= l && sup.i <= r) return sup;
         best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
         const Pair el = st[bl];
         if (el.i >= l) {
@@ -278,6 +286,9 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Pair *__
     } else {
         const int base = bl * bs;
         const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
+        // the same superset test over the sub-blocks jl..jr of this block
+        const Pair sup = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl, jr - jl + 1);
+        if (sup.i >= l && sup.i <= r) return sup;
         best = e.suf_s[l];
         right = e.pre_s[r];
         if (jr - jl > 1) {
