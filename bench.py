@@ -21,9 +21,10 @@ scans nothing and reads two 8-byte edge pairs instead of 4*S.  Summed over
 the queries that kernel processed, divided by its time measured with HIP
 events on the stream it runs on.  `traffic` is the HBM bytes per launch from
 rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/traffic.json, written by
---reduce-pmc), for exactly this workload or null.  An edge-table kernel is
-bound by random DRAM requests rather than bytes: `gather_ceiling` sets its
-random gathers per second against rmq_probe_gather's on the same GPU.
+--reduce-pmc), for exactly this workload or null.  The query kernels are
+bound by random reads (sparse-table entries, mostly from L2, and edge pairs
+from DRAM) rather than bytes: `gather_ceiling` sets their counts against
+rmq_probe_gather's random-read rates on the same GPU.
 
 cpu_baseline: the reference's HRMQ (RMQRMM64, built from the reference's own
 sources into oracle/_ref by oracle/Makefile) on this host's cores, OpenMP
@@ -95,8 +96,8 @@ class BlockLevel:
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
                       sub: BlockLevel | None = None) -> dict:
     """Per pass: bytes in the long and the short kernel, their query counts,
-    the fraction of cross-block edge parts computed, and the random gathers
-    of per-element edge pairs (with edge tables).  Q: (q, 2) int32 tensor on
+    the fraction of cross-block edge parts computed, the random gathers of
+    per-element edge pairs (with edge tables) and the sparse-table entries read.  Q: (q, 2) int32 tensor on
     the level's device.
 
     Short class (<= short_max elements, >= 32 with edge tables) and ranges
@@ -115,7 +116,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
-    n_long = n_short = parts = needed = gathers = 0
+    n_long = n_short = parts = needed = gathers = reads = 0
     sm = max(short_max, 32) if edges else short_max
     for c0 in range(0, Q.shape[0], 1 << 23):
         l = Q[c0:c0 + (1 << 23), 0].long()
@@ -135,6 +136,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             slow = ~((sup_i >= ls) & (sup_i <= rs))
             long_b += float((8 + 16 + slow.long() * (16 + 16 * (gr - gl > 1).long()) + out_bytes).sum())
             gathers += 2 * int(slow.sum())
+            reads += 2 * int(ls.numel()) + 2 * int((slow & (gr - gl > 1)).sum())
         else:
             long_b += float((8 + 4 * ln[same] + out_bytes).sum())
         x = ~same
@@ -143,6 +145,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         fast = (sup_i >= l) & (sup_i <= r)
         long_b += float((8 + 16 + out_bytes) * fast.sum())
         parts += 2 * int(fast.sum())
+        reads += 2 * int(l.numel())
         x = ~fast
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
@@ -157,6 +160,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         fv, fi = lvl.V[0, rB], lvl.I[0, rB]
         need_r = (fi > r) & (fv < cv)
         parts += 2 * int(l.numel())
+        reads += 2 * int(mid.sum()) + 2 * int(l.numel())
         n_need = int(need_l.sum() + need_r.sum())
         needed += n_need
         if edges:
@@ -166,7 +170,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             edge_b = 4 * need_l.long() * ((lB + 1) * bs - l) + 4 * need_r.long() * (r - rB * bs + 1)
         long_b += float((8 + 16 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
-            "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers}
+            "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads}
 
 
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
@@ -338,11 +342,11 @@ def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
     levels: host BlockLevel per block size."""
     per_dist = {}
-    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, random gathers]
+    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, edge-pair gathers, table reads]
     for lr in DISTS:
         e = bool(edges_for and edges_for[lr])
-        if e and bs_for[lr] % 32:
-            raise ValueError("the byte model needs block sizes that are multiples of 32 with edge tables")
+        if e and bs_for[lr] > 32 and bs_for[lr] % 32:
+            raise ValueError("the byte model needs block sizes of at most 32 or multiples of 32 with edge tables")
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
                                sub=levels.get(32) if e else None)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
@@ -366,22 +370,24 @@ def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=
                     else f"k_query_tiles<{opts_for[lr].long_lanes}, true>")
         short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
         per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
-        gathers = ab["gathers"]  # per-element edge pairs read (edge tables only)
-        for sym, b, t, g in ((long_sym, lb, lms, gathers), (short_sym, sb, sms, 0)):
-            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0])
+        for sym, b, t, g, rd in ((long_sym, lb, lms, ab["gathers"], ab["table_reads"]), (short_sym, sb, sms, 0, 0)):
+            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0, 0])
             acc[0] += b * len(steps_out)
             acc[1] += float(np.sum(t))
             acc[2] += len(steps_out)
             acc[3] += g * len(steps_out)
+            acc[4] += rd * len(steps_out)
     dom = max(kernels, key=lambda k: kernels[k][1])
     achieved = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
     per_kernel = {k: {"ms": v[1], "launches": v[2], "avg_launch_ms": v[1] / v[2],
                       "algorithmic_bytes_per_launch": v[0] / v[2],
                       "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
     for k, v in kernels.items():
-        if v[3]:
-            per_kernel[k]["random_gathers_per_launch"] = v[3] / v[2]
-            per_kernel[k]["random_gathers_per_s"] = v[3] / (v[1] / 1e3)
+        if v[3] or v[4]:
+            per_kernel[k]["edge_gathers_per_launch"] = v[3] / v[2]
+            per_kernel[k]["table_reads_per_launch"] = v[4] / v[2]
+            per_kernel[k]["edge_gathers_per_s"] = v[3] / (v[1] / 1e3)
+            per_kernel[k]["table_reads_per_s"] = v[4] / (v[1] / 1e3)
     return {"value": 3 * q * len(steps_out) / elapsed, "ms_per_step": elapsed / len(steps_out) * 1e3,
             "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
 
@@ -712,14 +718,20 @@ def main() -> int:
     for ix in indexes.values():
         ix.close()
     gather_ceiling = None
-    if "random_gathers_per_s" in tuned["per_kernel"][tuned["dominant"]] and not args.no_gather_probe:
-        gp = R.probe_gather(device=dev)
-        k_rate = tuned["per_kernel"][tuned["dominant"]]["random_gathers_per_s"]
-        gather_ceiling = {"gathers_per_s": gp["gathers_per_s"], "kernel_gathers_per_s": k_rate,
-                          "frac": k_rate / gp["gathers_per_s"],
-                          "probe": "rmq_probe_gather: 2^26 threads x 2 random 8 B gathers from two 1 GiB tables"}
-        log(f"bench: random-gather ceiling {gp['gathers_per_s']:.3e}/s; {tuned['dominant']} "
-            f"{k_rate:.3e}/s ({k_rate / gp['gathers_per_s']:.3f} of it)")
+    dk = tuned["per_kernel"][tuned["dominant"]]
+    if "table_reads_per_s" in dk and not args.no_gather_probe:
+        l2 = R.probe_gather(table_bytes=1 << 21, device=dev)["gathers_per_s"]
+        dram = R.probe_gather(table_bytes=1 << 30, device=dev)["gathers_per_s"]
+        t_bound = dk["table_reads_per_launch"] / l2 + dk["edge_gathers_per_launch"] / dram
+        gather_ceiling = {
+            "l2_table_reads_per_s": l2, "dram_gathers_per_s": dram,
+            "kernel_table_reads_per_s": dk["table_reads_per_s"], "kernel_edge_gathers_per_s": dk["edge_gathers_per_s"],
+            "frac": t_bound / (dk["avg_launch_ms"] / 1e3),
+            "probe": "rmq_probe_gather: 2^26 threads x 2 random 8 B reads from two 2 MiB tables (L2) and from two "
+                     "1 GiB tables (DRAM); frac = (table reads / L2 rate + edge-pair gathers / DRAM rate) / time"}
+        log(f"bench: random-read ceilings: L2 tables {l2:.3e}/s, DRAM {dram:.3e}/s; {tuned['dominant']} "
+            f"{dk['table_reads_per_s']:.3e} table reads/s + {dk['edge_gathers_per_s']:.3e} edge gathers/s "
+            f"= {gather_ceiling['frac']:.3f} of those rates")
 
     edges_for = {lr: picks[lr]["flags"] for lr in DISTS}
     workload = workload_key(n, q, bs_for, opts_for, edges_for)

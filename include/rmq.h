@@ -133,9 +133,9 @@ typedef struct rmq_tune_result {
 /* ---- autotuner (no counterpart in the reference, whose sweeps are shell
  *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
  *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^3 ..
- *      2^18), with and without edge tables (RMQ_BUILD_EDGES), then the
- *      short-range threshold and lanes per query at the best block size, and
- *      returns the fastest configuration (build it with rmq_build(..., bs,
+ *      2^18), with and without edge tables (RMQ_BUILD_EDGES), then, for the
+ *      four fastest of those, the short-range threshold and lanes per query,
+ *      and returns the fastest configuration (build it with rmq_build(..., bs,
  *      build_flags, ...)).  Results are cached
  *      per (n, length-distribution signature) in the text file cache_path
  *      (NULL: no cache).  d_val / d_idx are scratch outputs of q entries (one

@@ -65,6 +65,45 @@ __global__ __launch_bounds__(kWG) void k_block_minloc(const float *__restrict__ 
     }
 }
 
+// Small blocks (bs <= 64): one lane per block, strict '<' in index order
+// (leftmost); adjacent lanes read adjacent blocks.
+__global__ __launch_bounds__(kWG) void k_block_minloc_lane(const float *__restrict__ A, int n, int bs, int nb,
+                                                           Pair *__restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int lo = (int)((int64_t)b * bs);
+    const int hi = (int)min((int64_t)n - 1, (int64_t)lo + bs - 1);
+    float bv = A[lo];
+    int bi = lo;
+    for (int i = lo + 1; i <= hi; ++i) {
+        const float x = A[i];
+        if (x < bv) {
+            bv = x;
+            bi = i;
+        }
+    }
+    out[b] = Pair{bv, bi};
+}
+
+// Medium blocks (bs <= 8192): one wave per block.
+__global__ __launch_bounds__(kWG) void k_block_minloc_wave(const float *__restrict__ A, int n, int bs, int nb,
+                                                           Pair *__restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t waves = (int64_t)gridDim.x * kWavesPerWG;
+    for (int64_t b = (int64_t)blockIdx.x * kWavesPerWG + threadIdx.x / kWave; b < nb; b += waves) {
+        const int lo = (int)(b * bs);
+        const int hi = (int)min((int64_t)n - 1, (int64_t)lo + bs - 1);
+        float bv = INFINITY;
+        int bi = INT_MAX;
+        scan_range<kWave, 2>(A, lo, hi, lane, bv, bi);
+        wave_reduce(bv, bi);
+        if (lane == 0) {
+            if (bv == INFINITY) bi = lo;
+            out[b] = Pair{bv, bi};
+        }
+    }
+}
+
 // Sparse-table level: cur[i] = leftmost min of prev[i], prev[i+half].  The two
 // windows are disjoint and the left one holds the smaller indices, so the
 // left wins ties.
@@ -495,8 +534,15 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     int rc = hip_rc(hipGetDevice(&h->device));
     if (rc == RMQ_OK) rc = hip_rc(hipMalloc(&h->st, (size_t)h->st_bytes));
     if (rc == RMQ_OK) {
-        const int grid = (int)std::min<int64_t>(nb, 65535);
-        k_block_minloc<<<grid, kWG, 0, s>>>(d_A, (int)n, bs, (int)nb, h->st);
+        if (bs <= 64) {
+            k_block_minloc_lane<<<(unsigned)((nb + kWG - 1) / kWG), kWG, 0, s>>>(d_A, (int)n, bs, (int)nb, h->st);
+        } else if (bs <= 8192) {
+            const int grid = (int)std::min<int64_t>((nb + kWavesPerWG - 1) / kWavesPerWG, (int64_t)device_cus() * 16);
+            k_block_minloc_wave<<<grid, kWG, 0, s>>>(d_A, (int)n, bs, (int)nb, h->st);
+        } else {
+            const int grid = (int)std::min<int64_t>(nb, 65535);
+            k_block_minloc<<<grid, kWG, 0, s>>>(d_A, (int)n, bs, (int)nb, h->st);
+        }
         rc = hip_rc(hipGetLastError());
     }
     for (int32_t k = 1; rc == RMQ_OK && k < levels; ++k) {
@@ -820,11 +866,18 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, bs <= 2048 ? 16 : kWave};
         return o;
     };
-    float best = 1e30f;
-    int32_t best_bs = 0;
-    uint32_t best_flags = 0;
+    constexpr int kReps = 5;     // timed passes per configuration (best kept)
+    constexpr int kFinalists = 4;  // block-size candidates that get the threshold search
     int tried = 0;
-    // stage 1: block size, with and without edge tables, default thresholds
+    // stage 1: block size, with and without edge tables, default lanes; each
+    // at a small and at the default short-range threshold (with the default
+    // alone, short ranges would all be scanned and hide the block size)
+    struct Cand {
+        float ms;
+        int32_t bs;
+        uint32_t flags;
+    };
+    std::vector<Cand> stage1;
     for (int32_t bs : cands) {
         if (bs <= 0 || bs > n) continue;
         for (uint32_t flags : {0u, (uint32_t)RMQ_BUILD_EDGES}) {
@@ -832,53 +885,72 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             rc = rmq_build(d_A, n, bs, flags, s, &h);
             if (rc == RMQ_ERR_OOM && flags) continue;  // no room for edge tables: plain only
             if (rc != RMQ_OK) return rc;
-            float ms = 0.f;
-            rc = time_pass(h, d_Q, q, d_val, d_idx, base_opts(bs), s, 3, &ms);
+            float ms = 1e30f;
+            for (int32_t sm : {32, kDefaultShortMax}) {
+                rmq_query_opts_t o = base_opts(bs);
+                o.short_max = sm;
+                float t = 0.f;
+                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &t)) != RMQ_OK) break;
+                ++tried;
+                ms = std::min(ms, t);
+            }
             rmq_free(h);
             if (rc != RMQ_OK) return rc;
-            ++tried;
-            if (ms < best) {
-                best = ms;
-                best_bs = bs;
-                best_flags = flags;
-            }
+            stage1.push_back(Cand{ms, bs, flags});
         }
     }
-    if (best_bs == 0) return RMQ_ERR_INVALID_ARG;
-    rmq_handle_t *h = nullptr;
-    if ((rc = rmq_build(d_A, n, best_bs, best_flags, s, &h)) != RMQ_OK) return rc;
-    rmq_query_opts_t best_o = base_opts(best_bs);
-    // stage 2: short-range threshold and lanes per short query
-    for (int32_t sm : {32, 128, 512, 1024, 4096}) {
-        for (int32_t g : {4, 8, 16}) {
-            rmq_query_opts_t o = best_o;
-            o.short_max = sm;
-            o.group_lanes = g;
+    if (stage1.empty()) return RMQ_ERR_INVALID_ARG;
+    std::sort(stage1.begin(), stage1.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (stage1.size() > (size_t)kFinalists) stage1.resize(kFinalists);
+    // stages 2 and 3 for each finalist (defaults can favour a block size that
+    // is not the best once the thresholds are tuned): short-range threshold
+    // and lanes per short query, then lanes per long query
+    float best = 1e30f;
+    int32_t best_bs = 0;
+    uint32_t best_flags = 0;
+    rmq_query_opts_t best_o{};
+    for (const Cand &c : stage1) {
+        rmq_handle_t *h = nullptr;
+        if ((rc = rmq_build(d_A, n, c.bs, c.flags, s, &h)) != RMQ_OK) return rc;
+        rmq_query_opts_t cand_o = base_opts(c.bs);
+        float cand_ms = 1e30f;
+        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, cand_o, s, kReps, &cand_ms)) == RMQ_OK) ++tried;
+        for (int32_t sm : {32, 128, 512, 1024, 4096}) {
+            for (int32_t g : {4, 8, 16}) {
+                if (rc != RMQ_OK) break;
+                rmq_query_opts_t o = cand_o;
+                o.short_max = sm;
+                o.group_lanes = g;
+                float ms = 0.f;
+                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms)) != RMQ_OK) break;
+                ++tried;
+                if (ms < cand_ms) {
+                    cand_ms = ms;
+                    cand_o = o;
+                }
+            }
+        }
+        for (int32_t g : {4, 8, 16, 64}) {
+            if (rc != RMQ_OK) break;
+            rmq_query_opts_t o = cand_o;
+            o.long_lanes = g;
             float ms = 0.f;
-            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 3, &ms)) != RMQ_OK) break;
+            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms)) != RMQ_OK) break;
             ++tried;
-            if (ms < best) {
-                best = ms;
-                best_o = o;
+            if (ms < cand_ms) {
+                cand_ms = ms;
+                cand_o = o;
             }
         }
-        if (rc != RMQ_OK) break;
-    }
-    // stage 3: lanes per long query
-    for (int32_t g : {4, 8, 16, 64}) {
-        if (rc != RMQ_OK) break;
-        rmq_query_opts_t o = best_o;
-        o.long_lanes = g;
-        float ms = 0.f;
-        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, 3, &ms)) != RMQ_OK) break;
-        ++tried;
-        if (ms < best) {
-            best = ms;
-            best_o = o;
+        rmq_free(h);
+        if (rc != RMQ_OK) return rc;
+        if (cand_ms < best) {
+            best = cand_ms;
+            best_bs = c.bs;
+            best_flags = c.flags;
+            best_o = cand_o;
         }
     }
-    rmq_free(h);
-    if (rc != RMQ_OK) return rc;
     out->bs = best_bs;
     out->build_flags = best_flags;
     out->opts = best_o;

@@ -319,6 +319,23 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage) {
     constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
+    if constexpr (LONG) {
+        // One lane per query first: a range across blocks whose blocks'
+        // minimum has its argmin inside the range is answered here by two
+        // table reads; only the others go through the G-lane path.
+        bool done = false;
+        if ((mk >> lane) & 1) {
+            const int lB = block_of(m.l, magic), rB = block_of(m.r, magic);
+            if (lB != rB) {
+                const Pair sup = table_min(st, nb, lB, rB - lB + 1);
+                if (sup.i >= m.l && sup.i <= m.r) {
+                    write_result(out_v, out_i, qbase + lane, sup.v, sup.i);
+                    done = true;
+                }
+            }
+        }
+        mk &= ~(uint64_t)__ballot(done);
+    }
     if constexpr (G == kWave) {
         for (; mk; mk &= mk - 1) {
             const int src = __ffsll((unsigned long long)mk) - 1;
