@@ -93,8 +93,29 @@ class BlockLevel:
         return torch.where(t, yv, xv), torch.where(t, yi, xi)
 
 
+def left_part_min(A, sub, bs: int, l):
+    """Minimum value of A[l .. end of l's block] (the edge pair suf_b[l]):
+    the rest of l's 32-element sub-block, then the sub-blocks after it in the
+    block.  A: the array tensor; sub: its 32-element level; bs <= 32 or a
+    multiple of 32."""
+    import torch
+    n = A.numel()
+    end = torch.clamp((l // bs + 1) * bs, max=n) - 1
+    sub_end = end if bs <= 32 else torch.minimum(end, l // 32 * 32 + 31)  # at most 32 elements from l
+    k = torch.arange(32, device=l.device)
+    idx = l[:, None] + k[None, :]
+    vals = torch.where(idx <= sub_end[:, None], A[torch.clamp(idx, max=n - 1)], float("inf"))
+    v = vals.min(dim=1).values
+    g0, g1 = l // 32 + 1, end // 32
+    more = g1 >= g0
+    if bs > 32 and bool(more.any()):
+        mv, _ = sub.range_min(g0[more], g1[more] - g0[more] + 1)
+        v[more] = torch.minimum(v[more], mv)
+    return v
+
+
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
-                      sub: BlockLevel | None = None) -> dict:
+                      sub: BlockLevel | None = None, A=None) -> dict:
     """Per pass: bytes in the long and the short kernel, their query counts,
     the fraction of cross-block edge parts computed, the random gathers of
     per-element edge pairs (with edge tables) and the sparse-table entries read.  Q: (q, 2) int32 tensor on
@@ -111,8 +132,10 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     block's leftmost argmin lies inside it (then that entry is exact) or its
     block minimum cannot win (left: > the best so far; right: >= it); a needed
     part costs 4 B per element scanned, or 8 B (one edge pair) with edge
-    tables.  T = 16 when whole blocks (or sub-blocks) lie between.  sub: the
-    32-element level (block sizes that are multiples of 32), with edge tables."""
+    tables.  With edge tables the left pair, once read, counts in the best so
+    far before the right part is decided; a scanned left part (spread over a
+    lane group) does not.  T = 16 when whole blocks (or sub-blocks) lie
+    between.  sub: the 32-element level and A the array, with edge tables."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
@@ -157,6 +180,8 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         exact_l = ei >= l
         need_l = ~exact_l & (ev <= mv)
         cv = torch.where(exact_l & ((ev < mv) | ((ev == mv) & (ei < mi))), ev, mv)
+        if edges and bool(need_l.any()):
+            cv[need_l] = torch.minimum(cv[need_l], left_part_min(A, sub, bs, l[need_l]))
         fv, fi = lvl.V[0, rB], lvl.I[0, rB]
         need_r = (fi > r) & (fv < cv)
         parts += 2 * int(l.numel())
@@ -338,9 +363,9 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
     return steps_out, time.perf_counter() - t_start
 
 
-def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None) -> dict:
+def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None) -> dict:
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
-    levels: host BlockLevel per block size."""
+    levels: BlockLevel per block size (and 32, with edge tables)."""
     per_dist = {}
     kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, edge-pair gathers, table reads]
     for lr in DISTS:
@@ -348,7 +373,7 @@ def summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=
         if e and bs_for[lr] > 32 and bs_for[lr] % 32:
             raise ValueError("the byte model needs block sizes of at most 32 or multiples of 32 with edge tables")
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
-                               sub=levels.get(32) if e else None)
+                               sub=levels.get(32) if e else None, A=A)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]
@@ -681,7 +706,7 @@ def main() -> int:
                                            {lr: fixed_opts for lr in DISTS}, Q_dev, out_v, out_i,
                                            args.fixed_steps, 1, f"bs={args.bs}")
     levels = {args.bs: BlockLevel(A, args.bs)}
-    fixed = summarise(Q_dev, q, fixed_steps, fixed_elapsed, levels, {lr: args.bs for lr in DISTS},
+    fixed = summarise(A, Q_dev, q, fixed_steps, fixed_elapsed, levels, {lr: args.bs for lr in DISTS},
                       {lr: fixed_opts for lr in DISTS})
     fixed_index.close()
 
@@ -711,7 +736,7 @@ def main() -> int:
                                    "tuned")
     for b in (set(bs_for.values()) | ({32} if any(p["flags"] for p in picks.values()) else set())) - set(levels):
         levels[b] = BlockLevel(A, b)
-    tuned = summarise(Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
+    tuned = summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
                       {lr: picks[lr]["flags"] for lr in DISTS})
     log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")

@@ -1,0 +1,168 @@
+"""The rules that let a long range skip its edge blocks, checked by brute force.
+
+The kernels answer a range [l, r] across blocks lB < rB from block minima
+(each block's min and leftmost argmin) before touching any element:
+  1. superset: the minimum of blocks lB..rB is the answer when its argmin lies
+     in [l, r];
+  2. exact edge: block lB's entry is the left part's answer when its argmin is
+     >= l; block rB's entry is the right part's when its argmin is <= r;
+  3. skipped edge: the left part is needed only when block lB's minimum is <=
+     the best so far, the right part only when block rB's is strictly below.
+The same rules run over 32-element sub-blocks inside one block.  Here they are
+restated on plain Python lists and compared with a brute-force leftmost
+minimum over every (l, r) of small arrays with ties and +inf, then bench.py's
+byte model (which replays the same decisions on tensors) is compared with the
+restatement's counts.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402
+
+INF, IMAX = math.inf, 2 ** 31 - 1
+
+
+def better(a, b):
+    return a[0] < b[0] or (a[0] == b[0] and a[1] < b[1])
+
+
+def lexmin(a, b):
+    return a if better(a, b) else b
+
+
+def brute(A, l, r):
+    best = (INF, IMAX)
+    for i in range(l, r + 1):
+        best = lexmin((float(A[i]), i), best)
+    return best
+
+
+def block_mins(A, bs):
+    """(min, leftmost argmin) per block; an all-+inf block gives its start."""
+    return [brute(A, b, min(b + bs, len(A)) - 1) for b in range(0, len(A), bs)]
+
+
+def lazy(A, bs, bm, l, r, left_counts=True):
+    """(answer, parts computed, superset hit) for lB < rB, following rules
+    1-3.  left_counts: a computed left part joins the best so far before the
+    right part is decided (the edge-table path reads it as one pair); False
+    for the scan path, whose left scan is spread over a lane group."""
+    lB, rB = l // bs, r // bs
+    sup = (INF, IMAX)
+    for b in range(lB, rB + 1):
+        sup = lexmin(bm[b], sup)
+    if l <= sup[1] <= r:
+        return sup, 0, True
+    best = (INF, IMAX)
+    for b in range(lB + 1, rB):
+        best = lexmin(bm[b], best)
+    need = 0
+    left = (INF, IMAX)
+    el = bm[lB]
+    if el[1] >= l:
+        best = lexmin(el, best)
+    elif el[0] <= best[0]:
+        left = brute(A, l, lB * bs + bs - 1)
+        need += 1
+        if left_counts:
+            best = lexmin(left, best)
+    er = bm[rB]
+    if er[1] <= r:
+        best = lexmin(er, best)
+    elif er[0] < best[0]:
+        best = lexmin(brute(A, rB * bs, r), best)
+        need += 1
+    return lexmin(left, best), need, False
+
+
+def arrays(rng):
+    out = []
+    for n in (2, 7, 33, 64, 97):
+        out += [rng.integers(0, 3, n).astype(np.float32), rng.integers(0, 16, n).astype(np.float32),
+                np.full(n, 5.0, np.float32), np.arange(n, dtype=np.float32),
+                (1000 - np.arange(n)).astype(np.float32), np.full(n, np.inf, np.float32)]
+        x = rng.integers(0, 4, n).astype(np.float32)
+        x[rng.random(n) < 0.5] = np.inf
+        out.append(x)
+    return out
+
+
+@pytest.mark.parametrize("left_counts", [True, False])
+@pytest.mark.parametrize("bs", [1, 2, 3, 8, 16, 33])
+def test_rules_give_the_leftmost_minimum(bs, left_counts):
+    for A in arrays(np.random.default_rng(bs)):
+        bm = block_mins(A, bs)
+        for l in range(len(A)):
+            for r in range(l, len(A)):
+                if l // bs == r // bs:
+                    continue
+                got = lazy(A, bs, bm, l, r, left_counts)[0]
+                exp = brute(A, l, r)
+                assert got[1] == exp[1] and np.float32(got[0]).tobytes() == np.float32(exp[0]).tobytes(), \
+                    (A.tolist(), bs, l, r, got, exp)
+
+
+def test_superset_rule_inside_one_block_over_sub_blocks():
+    """Sub-blocks of 32 restart at each block start (bs a multiple of 32 here)."""
+    rng = np.random.default_rng(7)
+    for A in arrays(rng) + [rng.integers(0, 8, 300).astype(np.float32)]:
+        n = len(A)
+        for bs in (64, 96):
+            for l in range(n):
+                for r in range(l, min(n, l + 100)):
+                    if l // bs != r // bs:
+                        continue
+                    base = l // bs * bs
+                    lo = base + (l - base) // 32 * 32
+                    hi = min(base + (r - base) // 32 * 32 + 31, min(base + bs, n) - 1)
+                    sup = brute(A, lo, hi)
+                    if l <= sup[1] <= r:
+                        assert sup == brute(A, l, r), (bs, l, r)
+
+
+@pytest.mark.parametrize("edges", [False, True])
+def test_byte_model_replays_the_rules(edges):
+    rng = np.random.default_rng(11 + edges)
+    for trial in range(16):
+        n = int(rng.integers(100, 2500))
+        bs = int(rng.choice([8, 24, 64, 128, 256] if edges else [4, 16, 37, 64, 256]))
+        A = rng.integers(0, int(rng.choice([3, 50, 10 ** 6])), n).astype(np.float32)
+        if trial % 4 == 0 and not edges:
+            A[rng.random(n) < 0.3] = np.inf
+        Q = []
+        for _ in range(300):
+            l = int(rng.integers(0, n))
+            r = int(rng.integers(l, min(n, l + int(rng.choice([40, 300, n])))))
+            Q.append((l, r))
+        Q = np.array(Q, np.int32)
+        bm = block_mins(A, bs)
+        parts = need = gathers = 0
+        for l, r in Q:
+            if r - l + 1 <= 32:
+                continue
+            lB, rB = l // bs, r // bs
+            if lB == rB:
+                if edges:
+                    base = lB * bs
+                    lo = base + (l - base) // 32 * 32
+                    hi = min(base + (r - base) // 32 * 32 + 31, min(base + bs, n) - 1)
+                    s = brute(A, lo, hi)
+                    gathers += 0 if l <= s[1] <= r else 2
+                continue
+            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges)
+            parts += 2
+            need += k
+            gathers += k if edges else 0
+        lvl = bench.BlockLevel(torch.from_numpy(A), bs)
+        sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
+        ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, 32, edges=edges, sub=sub, A=torch.from_numpy(A))
+        assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
+        assert ab["gathers"] == gathers
+        for k in range(lvl.nb):  # block minima: leftmost argmin, all-+inf block -> its start
+            assert int(lvl.I[0, k]) == bm[k][1]
