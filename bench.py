@@ -93,6 +93,24 @@ class BlockLevel:
         return torch.where(t, yv, xv), torch.where(t, yi, xi)
 
 
+HIER_MIN = 4  # kHierMin: whole sub-blocks between the ends before a scan reads their minima
+
+
+def scan_bytes(lo, hi, bs: int):
+    """Bytes of the scan of [lo, hi] inside one block: element by element, or,
+    with at least HIER_MIN whole 32-element sub-blocks between the ends (and
+    bs > 32), the partial sub-blocks element by element and the whole ones as
+    8-byte (min, argmin) pairs."""
+    import torch
+    direct = 4 * (hi - lo + 1)
+    if bs <= 32:
+        return direct
+    base = lo // bs * bs
+    jl, jr = (lo - base) // 32, (hi - base) // 32
+    hier = 4 * (base + jl * 32 + 32 - lo) + 8 * (jr - jl - 1) + 4 * (hi - (base + jr * 32) + 1)
+    return torch.where(jr - jl - 1 >= HIER_MIN, hier, direct)
+
+
 def left_part_min(A, sub, bs: int, l):
     """Minimum value of A[l .. end of l's block] (the edge pair suf_b[l]):
     the rest of l's 32-element sub-block, then the sub-blocks after it in the
@@ -121,8 +139,10 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     per-element edge pairs (with edge tables) and the sparse-table entries read.  Q: (q, 2) int32 tensor on
     the level's device.
 
-    Short class (<= short_max elements, >= 32 with edge tables) and ranges
-    inside one block without edge tables: B = 8 + 4*len + out.  Every other
+    Short class (<= short_max elements, >= 32 with edge tables): B = 8 +
+    4*len + out.  Ranges inside one block without edge tables: B = 8 + the
+    scan's bytes (scan_bytes: 4 per element, or 8 per whole 32-element
+    sub-block between the ends for long ones) + out.  Every other
     range first reads the minimum of the blocks it touches (of the 32-element
     sub-blocks it touches, inside one block; two table entries, 16 B): when
     its argmin lies in [l, r] that is the answer, B = 8 + 16 + out.
@@ -131,8 +151,8 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     edge part the query needs, where an edge part is needed unless its
     block's leftmost argmin lies inside it (then that entry is exact) or its
     block minimum cannot win (left: > the best so far; right: >= it); a needed
-    part costs 4 B per element scanned, or 8 B (one edge pair) with edge
-    tables.  With edge tables the left pair, once read, counts in the best so
+    part costs its scan's bytes (scan_bytes), or 8 B (one edge pair) with
+    edge tables.  With edge tables the left pair, once read, counts in the best so
     far before the right part is decided; a scanned left part (spread over a
     lane group) does not.  T = 16 when whole blocks (or sub-blocks) lie
     between.  sub: the 32-element level and A the array, with edge tables."""
@@ -161,7 +181,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             gathers += 2 * int(slow.sum())
             reads += 2 * int(ls.numel()) + 2 * int((slow & (gr - gl > 1)).sum())
         else:
-            long_b += float((8 + 4 * ln[same] + out_bytes).sum())
+            long_b += float((8 + scan_bytes(l[same], r[same], bs) + out_bytes).sum())
         x = ~same
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         sup_i = lvl.range_min(lB, rB - lB + 1)[1]
@@ -192,7 +212,8 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             edge_b = 8 * need_l.long() + 8 * need_r.long()
             gathers += n_need
         else:
-            edge_b = 4 * need_l.long() * ((lB + 1) * bs - l) + 4 * need_r.long() * (r - rB * bs + 1)
+            edge_b = (need_l.long() * scan_bytes(l, (lB + 1) * bs - 1, bs)
+                      + need_r.long() * scan_bytes(rB * bs, r, bs))
         long_b += float((8 + 16 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
             "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads}

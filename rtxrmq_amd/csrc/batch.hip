@@ -24,15 +24,12 @@
 namespace rmqx {
 
 struct ArrayDesc {
-    const float *A;
-    const Pair *st;
-    uint64_t magic;  // block_magic(bs)
-    int32_t n, bs, nb, pad;
+    Blocks k;
     EdgeTables edges;  // pre_b == nullptr: no edge tables
 };
 
 inline ArrayDesc desc_of(const rmq_handle *h) {
-    ArrayDesc d{h->A, h->st, block_magic(h->bs), (int32_t)h->n, h->bs, (int32_t)h->nb, 0, h->edges};
+    ArrayDesc d{blocks_of(h), h->edges};
     return d;
 }
 
@@ -68,17 +65,15 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
         const int sm = edges ? max(short_max, kSub) : short_max;
         const int64_t tiles = ((int64_t)w.count + kWave - 1) / kWave;
         for (int64_t t = 0; t < tiles; ++t) {
-            const TileMasks m = load_tile(w.Q, w.count, d.n, sm, t, lane, true, w.out_v, w.out_i);
-            answer_class<GS, false>(d.A, d.bs, d.magic, d.nb, d.st, m.shorts, m, lane, t * kWave, w.out_v,
-                                    w.out_i, stage[wv]);
+            const TileMasks m = load_tile(w.Q, w.count, d.k.n, sm, t, lane, true, w.out_v, w.out_i);
+            answer_class<GS, false>(d.k, m.shorts, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
             if (edges) {
                 if ((m.longs >> lane) & 1) {
-                    const Pair b = answer_edges(d.edges, d.st, d.nb, d.bs, d.magic, m.l, m.r);
+                    const Pair b = answer_edges(d.edges, d.k, m.l, m.r);
                     write_result(w.out_v, w.out_i, t * kWave + lane, b.v, b.i);
                 }
             } else {
-                answer_class<GL, true>(d.A, d.bs, d.magic, d.nb, d.st, m.longs, m, lane, t * kWave, w.out_v,
-                                       w.out_i, stage[wv]);
+                answer_class<GL, true>(d.k, m.longs, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
             }
         }
     }
@@ -702,6 +697,7 @@ RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t 
     while (((int64_t)1 << levels) <= nb) ++levels;
     int64_t table = (int64_t)sizeof(Pair) * nb * levels;
     if (flags & RMQ_BUILD_EDGES) table += (int64_t)sizeof(Pair) * (4 * n + n / 2);  // edge tables, generous
+    else if (bs > kSub) table += (int64_t)sizeof(Pair) * nb * ((bs + kSub - 1) / kSub);  // sub-block minima
     size_t free_b = 0, total_b = 0;
     int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
     if (rc != RMQ_OK) return rc;

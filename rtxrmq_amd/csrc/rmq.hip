@@ -232,18 +232,17 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
 
 // Ranges of more than short_max (>= 32) elements with edge tables: one lane
 // per query, four O(1) lookups (answer_edges), no scan and no reduction.
-__global__ __launch_bounds__(kWG) void k_query_edges(int n, int bs, uint64_t magic, int nb,
-                                                     const Pair *__restrict__ st, EdgeTables e,
-                                                     const int2 *__restrict__ Q, int64_t q, int short_max,
-                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+__global__ __launch_bounds__(kWG) void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
+                                                     int short_max, float *__restrict__ out_v,
+                                                     int32_t *__restrict__ out_i) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, false, out_v, out_i);
+        const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, false, out_v, out_i);
         if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges(e, st, nb, bs, magic, m.l, m.r);
+            const Pair best = answer_edges(e, k, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
     }
@@ -254,18 +253,16 @@ __global__ __launch_bounds__(kWG) void k_query_edges(int n, int bs, uint64_t mag
 // answer_class.  The short-class kernel also writes (NaN, -1) for invalid
 // queries.  Persistent grid, static stride over tiles.
 template <int G, bool LONG>
-__global__ __launch_bounds__(kWG) void k_query_tiles(const float *__restrict__ A, int n, int bs, uint64_t magic, int nb,
-                                                     const Pair *__restrict__ st, const int2 *__restrict__ Q,
-                                                     int64_t q, int short_max, float *__restrict__ out_v,
-                                                     int32_t *__restrict__ out_i) {
+__global__ __launch_bounds__(kWG) void k_query_tiles(Blocks k, const int2 *__restrict__ Q, int64_t q, int short_max,
+                                                     float *__restrict__ out_v, int32_t *__restrict__ out_i) {
     __shared__ TileStage stage[kWavesPerWG];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, n, short_max, t, lane, !LONG, out_v, out_i);
-        answer_class<G, LONG>(A, bs, magic, nb, st, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
+        const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, !LONG, out_v, out_i);
+        answer_class<G, LONG>(k, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
                               stage[wv]);
     }
 }
@@ -362,17 +359,12 @@ int tiles_grid(int64_t q, int per_cu) {
 template <bool LONG>
 int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int grid, int short_max, float *val,
                  int32_t *idx, hipStream_t s) {
-    const int n = (int)h->n, bs = h->bs, nb = (int)h->nb;
-    const uint64_t mg = block_magic(bs);
+    const Blocks k = blocks_of(h);
     switch (lanes) {
-    case 4: k_query_tiles<4, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx); break;
-    case 8: k_query_tiles<8, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx); break;
-    case 16:
-        k_query_tiles<16, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx);
-        break;
-    default:
-        k_query_tiles<64, LONG><<<grid, kWG, 0, s>>>(h->A, n, bs, mg, nb, h->st, Q, q, short_max, val, idx);
-        break;
+    case 4: k_query_tiles<4, LONG><<<grid, kWG, 0, s>>>(k, Q, q, short_max, val, idx); break;
+    case 8: k_query_tiles<8, LONG><<<grid, kWG, 0, s>>>(k, Q, q, short_max, val, idx); break;
+    case 16: k_query_tiles<16, LONG><<<grid, kWG, 0, s>>>(k, Q, q, short_max, val, idx); break;
+    default: k_query_tiles<64, LONG><<<grid, kWG, 0, s>>>(k, Q, q, short_max, val, idx); break;
     }
     return hip_rc(hipGetLastError());
 }
@@ -395,7 +387,7 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
     int grid = tiles_grid(q, 8);
     if (o.long_wgs > 0) grid = (int)std::min<int64_t>(o.long_wgs, grid);
     if (h->edges.pre_b) {
-        k_query_edges<<<grid, kWG, 0, s>>>((int)h->n, h->bs, block_magic(h->bs), (int)h->nb, h->st, h->edges, Q, q,
+        k_query_edges<<<grid, kWG, 0, s>>>(blocks_of(h), h->edges, Q, q,
                                            effective_short_max(h, o), val, idx);
         return hip_rc(hipGetLastError());
     }
@@ -554,9 +546,29 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     }
     h->flags = flags;
     if (rc == RMQ_OK && (flags & RMQ_BUILD_EDGES)) rc = build_edges(h, s);
+    // sub-block minima (32 elements, restarting at every block start) for
+    // long scans: row 0 of the edge tables' sub-block table when built, else
+    // an allocation of their own, 8 bytes per 32 elements
+    if (rc == RMQ_OK && bs > kSub) {
+        h->nsub = (bs + kSub - 1) / kSub;
+        if (h->edges.st_s) {
+            h->sub = const_cast<Pair *>(h->edges.st_s);
+        } else {
+            const int64_t subs = nb * h->nsub;
+            h->sub_bytes = (int64_t)sizeof(Pair) * subs;
+            rc = hip_rc(hipMalloc(&h->sub_mem, (size_t)h->sub_bytes));
+            if (rc == RMQ_OK) {
+                h->sub = static_cast<Pair *>(h->sub_mem);
+                k_sub_edges<<<(unsigned)((subs * kSub + kWG - 1) / kWG), kWG, 0, s>>>(d_A, (int)n, bs, h->nsub, subs,
+                                                                                     nullptr, nullptr, h->sub);
+                rc = hip_rc(hipGetLastError());
+            }
+        }
+    }
     if (rc != RMQ_OK) {
         if (h->st) (void)hipFree(h->st);
         if (h->edge_mem) (void)hipFree(h->edge_mem);
+        if (h->sub_mem) (void)hipFree(h->sub_mem);
         delete h;
         return rc;
     }
@@ -604,6 +616,7 @@ RMQ_API void rmq_free(rmq_handle_t *h) {
     if (!h) return;
     if (h->st) (void)hipFree(h->st);
     if (h->edge_mem) (void)hipFree(h->edge_mem);
+    if (h->sub_mem) (void)hipFree(h->sub_mem);
     delete h;
 }
 
@@ -614,7 +627,7 @@ RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int6
     if (bs) *bs = h->bs;
     if (nb) *nb = h->nb;
     if (levels) *levels = h->levels;
-    if (bytes) *bytes = h->st_bytes + h->edge_bytes;
+    if (bytes) *bytes = h->st_bytes + h->edge_bytes + h->sub_bytes;
     return RMQ_OK;
 }
 

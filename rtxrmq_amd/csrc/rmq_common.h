@@ -21,6 +21,9 @@ constexpr int kWavesPerWG = kWG / kWave;
 constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
 constexpr int kDefaultShortMax = 1024;
 constexpr int kDefaultGroupLanes = 16;
+constexpr int kSub = 32;  // sub-block size (elements); sub-blocks restart at every block start
+constexpr int kSubShift = 5;
+constexpr int kHierMin = 4;  // whole sub-blocks between the ends before a scan reads their minima instead
 
 struct __align__(8) Pair {
     float v;
@@ -165,13 +168,6 @@ __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
 }
 
-// One query [l, r] answered by the G lanes of a group (gl = lane in group).
-// Short class: one direct scan.  Long class: lB == rB is one in-block scan
-// (src/rtx_kernels.cu:116-135); otherwise the left edge block from l and the
-// right edge block up to r (:151-177) are scanned and, when whole blocks lie
-// between, the sparse table over block minima (:138-149) gives their leftmost
-// minimum in O(1).  The group's result is combined with the table's with the
-// index as tie-break, so the leftmost minimum wins.
 // Block index of element e: e / bs by one multiply-high with the handle's
 // magic = floor((2^64-1)/bs) + 1 (exact for 0 <= e < 2^31 and bs >= 2);
 // magic 0 means bs == 1.
@@ -191,28 +187,81 @@ __device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t s
     return (y.v < x.v) ? y : x;
 }
 
+// Everything the query kernels read of one built array.
+struct Blocks {
+    const float *A;
+    const Pair *st;   // levels rows of nb pairs; row 0 = each block's (min, leftmost argmin)
+    const Pair *sub;  // nb * nsub pairs: each 32-element sub-block's (min, leftmost argmin); null when bs <= 32
+    uint64_t magic;   // block_magic(bs)
+    int n, bs, nb, nsub;
+};
+
+// Scans pairs P[a..b] (each a (min, leftmost argmin) of consecutive elements,
+// in index order) with STRIDE lanes, U loads per lane in flight.
+template <int STRIDE, int U>
+__device__ __forceinline__ void scan_pairs(const Pair *__restrict__ P, int64_t a, int64_t b, int lane, float &bv,
+                                           int &bi) {
+    int64_t j = a + lane;
+    for (; j + (U - 1) * STRIDE <= b; j += U * STRIDE) {
+        Pair x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = P[j + u * STRIDE];
+#pragma unroll
+        for (int u = 0; u < U; ++u) take_if_better(x[u].v, x[u].i, bv, bi);
+    }
+    for (; j <= b; j += STRIDE) take_if_better(P[j].v, P[j].i, bv, bi);
+}
+
+// Scans [lo, hi] inside block b.  With a sub-block level and at least
+// kHierMin whole sub-blocks between the ends, the partial sub-blocks at both
+// ends are scanned element by element and the whole ones between read as
+// their (min, argmin) pairs: 8 bytes per 32 elements.  Otherwise one direct
+// scan.
+template <int G, int U>
+__device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi, int gl, float &bv, int &bi) {
+    if (k.sub) {
+        const int base = b * k.bs;
+        const int jl = (lo - base) >> kSubShift, jr = (hi - base) >> kSubShift;
+        if (jr - jl - 1 >= kHierMin) {
+            const int64_t row = (int64_t)b * k.nsub;
+            scan_range<G, U>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
+            scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
+            scan_range<G, U>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
+            return;
+        }
+    }
+    scan_range<G, U>(k.A, lo, hi, gl, bv, bi);
+}
+
+// One query [l, r] answered by the G lanes of a group (gl = lane in group).
+// Short class: one direct scan.  Long class: lB == rB is one in-block scan
+// (src/rtx_kernels.cu:116-135), through the sub-block minima when it is long;
+// otherwise the blocks between give their leftmost minimum in O(1) from the
+// sparse table over block minima (:138-149), and the edge blocks (:151-177)
+// are taken from their minima or scanned only when they can still win.  The
+// group's partial results are combined with (mv, mi) with the index as
+// tie-break, so the leftmost minimum wins.
 template <int G, int U, bool LONG>
-__device__ __forceinline__ void answer_partial(const float *__restrict__ A, int bs, uint64_t magic, int nb,
-                                               const Pair *__restrict__ st, int l, int r, int gl, float &bv,
-                                               int &bi, float &mv, int &mi) {
+__device__ __forceinline__ void answer_partial(const Blocks &k, int l, int r, int gl, float &bv, int &bi, float &mv,
+                                               int &mi) {
     if constexpr (!LONG) {
-        scan_range<G, U>(A, l, r, gl, bv, bi);
+        scan_range<G, U>(k.A, l, r, gl, bv, bi);
     } else {
-        const int lB = block_of(l, magic), rB = block_of(r, magic);
+        const int lB = block_of(l, k.magic), rB = block_of(r, k.magic);
         if (lB == rB) {
-            scan_range<G, U>(A, l, r, gl, bv, bi);
+            scan_hier<G, U>(k, lB, l, r, gl, bv, bi);
         } else {
             // The minimum of the whole blocks lB..rB covers the range; when
             // its leftmost argmin lies inside [l, r] it is the answer (every
             // element before it in those blocks is larger).
-            const Pair sup = table_min(st, nb, lB, rB - lB + 1);
+            const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
             if (sup.i >= l && sup.i <= r) {
                 mv = sup.v;
                 mi = sup.i;
                 return;
             }
             if (rB - lB > 1) {
-                const Pair m = table_min(st, nb, lB + 1, rB - lB - 1);
+                const Pair m = table_min(k.st, k.nb, lB + 1, rB - lB - 1);
                 mv = m.v;
                 mi = m.i;
             }
@@ -221,12 +270,12 @@ __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int 
             // the block minimum, a lower bound for the part, cannot win; only
             // otherwise scanned.  (mv, mi) is known to every lane of the group,
             // so the group takes the same branch.
-            const Pair eb = st[lB];
+            const Pair eb = k.st[lB];
             if (eb.i >= l) take_if_better(eb.v, eb.i, mv, mi);
-            else if (eb.v <= mv) scan_range<G, U>(A, l, lB * bs + bs - 1, gl, bv, bi);  // earlier: wins ties
-            const Pair er = st[rB];
+            else if (eb.v <= mv) scan_hier<G, U>(k, lB, l, lB * k.bs + k.bs - 1, gl, bv, bi);  // earlier: wins ties
+            const Pair er = k.st[rB];
             if (er.i <= r) take_if_better(er.v, er.i, mv, mi);
-            else if (er.v < mv) scan_range<G, U>(A, rB * bs, r, gl, bv, bi);  // later: must be smaller
+            else if (er.v < mv) scan_hier<G, U>(k, rB, rB * k.bs, r, gl, bv, bi);  // later: must be smaller
         }
     }
 }
@@ -234,8 +283,6 @@ __device__ __forceinline__ void answer_partial(const float *__restrict__ A, int 
 // ---------------------------------------------------------------------------
 // edge tables (RMQ_BUILD_EDGES): precomputed edge-block searches
 // ---------------------------------------------------------------------------
-constexpr int kSub = 32;  // sub-block size of the in-block level (elements)
-constexpr int kSubShift = 5;
 
 struct EdgeTables {
     const Pair *pre_b;  // per element i: leftmost min of [start of i's block, i]
@@ -255,9 +302,10 @@ struct EdgeTables {
 // level down (sub-blocks of the block; a range of > 32 elements spans at
 // least two).  Parts are combined left to right with strict '<', so the
 // leftmost minimum wins.  One lane per query: no reduction needed.
-__device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Pair *__restrict__ st, int nb, int bs,
-                                             uint64_t magic, int l, int r) {
-    const int bl = block_of(l, magic), br = block_of(r, magic);
+__device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
+    const Pair *__restrict__ st = k.st;
+    const int nb = k.nb, bs = k.bs;
+    const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
     Pair best, right;
     if (bl != br) {
         // The minimum of blocks bl..br, when its leftmost argmin lies in
@@ -313,9 +361,7 @@ struct TileStage {
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
 template <int G, bool LONG>
-__device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs, uint64_t magic, int nb,
-                                             const Pair *__restrict__ st,
-                                             uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
+__device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage) {
     constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
@@ -325,9 +371,9 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
         // table reads; only the others go through the G-lane path.
         bool done = false;
         if ((mk >> lane) & 1) {
-            const int lB = block_of(m.l, magic), rB = block_of(m.r, magic);
+            const int lB = block_of(m.l, k.magic), rB = block_of(m.r, k.magic);
             if (lB != rB) {
-                const Pair sup = table_min(st, nb, lB, rB - lB + 1);
+                const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
                 if (sup.i >= m.l && sup.i <= m.r) {
                     write_result(out_v, out_i, qbase + lane, sup.v, sup.i);
                     done = true;
@@ -343,7 +389,7 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
             const int r = __builtin_amdgcn_readlane(m.r, src);
             float bv = INFINITY, mv = INFINITY;
             int bi = INT_MAX, mi = INT_MAX;
-            answer_partial<kWave, U, LONG>(A, bs, magic, nb, st, l, r, lane, bv, bi, mv, mi);
+            answer_partial<kWave, U, LONG>(k, l, r, lane, bv, bi, mv, mi);
             wave_reduce(bv, bi);
             take_if_better(mv, mi, bv, bi);
             if (lane == 0) {
@@ -356,10 +402,10 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
         const int g = lane / G;
         const int gl = lane & (G - 1);
         if ((mk >> lane) & 1) {
-            const uint32_t k = mask_rank(mk);
-            stage.l[k] = m.l;
-            stage.r[k] = m.r;
-            stage.lane[k] = lane;
+            const uint32_t rank = mask_rank(mk);
+            stage.l[rank] = m.l;
+            stage.r[rank] = m.r;
+            stage.lane[rank] = lane;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -372,7 +418,7 @@ __device__ __forceinline__ void answer_class(const float *__restrict__ A, int bs
             int ql = 0;
             if (has) {
                 ql = stage.l[s];
-                answer_partial<G, U, LONG>(A, bs, magic, nb, st, ql, stage.r[s], gl, bv, bi, mv, mi);
+                answer_partial<G, U, LONG>(k, ql, stage.r[s], gl, bv, bi, mv, mi);
             }
             group_reduce<G>(bv, bi);
             take_if_better(mv, mi, bv, bi);
@@ -427,9 +473,17 @@ struct rmq_handle {
     rmqx::EdgeTables edges;  // all pointers null unless built with RMQ_BUILD_EDGES
     void *edge_mem;          // one allocation holding every edge array
     int64_t edge_bytes;
+    rmqx::Pair *sub;         // sub-block minima (bs > 32): row 0 of edges.st_s, or sub_mem
+    int32_t nsub;            // sub-blocks per block, ceil(bs / 32)
+    void *sub_mem;           // own allocation when built without edge tables
+    int64_t sub_bytes;
 };
 
 namespace rmqx {
+
+inline Blocks blocks_of(const rmq_handle *h) {
+    return Blocks{h->A, h->st, h->sub, block_magic(h->bs), (int)h->n, h->bs, (int)h->nb, h->nsub};
+}
 
 inline int check_device(const rmq_handle *h) {
     int dev = -1;

@@ -60,6 +60,33 @@ def test_exhaustive_small(n, bs, flags, gpu):
     assert_exact(v, i, ev, ei, f"n={n} bs={bs} flags={flags}")
 
 
+@pytest.mark.parametrize("flags", FLAGS)
+@pytest.mark.parametrize("kind", ["ties4", "asc", "desc", "inf_runs"])
+@pytest.mark.parametrize("bs", [256, 777, 1000, 4096])
+def test_long_scans_through_sub_blocks(kind, bs, flags, gpu):
+    """Ranges inside one block and edge parts that must be scanned, read
+    through the 32-element sub-block minima (blocks that are and are not
+    multiples of 32, heavy ties, monotone data, runs of +inf)."""
+    n = 20011
+    rng = np.random.default_rng(bs + len(kind))
+    if kind == "ties4":
+        A = rng.integers(0, 4, n).astype(np.float32)
+    elif kind == "asc":
+        A = np.arange(n, dtype=np.float32)
+    elif kind == "desc":
+        A = (np.uint32(0x3F800000) - 1 - np.arange(n, dtype=np.uint32)).view(np.float32)
+    else:
+        A = rng.integers(0, 8, n).astype(np.float32)
+        A[(np.arange(n) // 300) % 3 == 1] = np.inf
+    l = rng.integers(0, n, 60000)
+    ln = np.exp2(rng.uniform(5, 14, l.size)).astype(np.int64)
+    r = np.minimum(l + ln - 1, n - 1)
+    Q = np.stack([l, r], 1).astype(np.int32)
+    ev, ei = O.rmq_brute(A, Q)
+    v, i = run_engine(A, Q, bs, gpu, flags=flags)
+    assert_exact(v, i, ev, ei, f"{kind} bs={bs} flags={flags}")
+
+
 @pytest.fixture(scope="module")
 def mixed_case():
     A = O.gen_array(200003, 11, "ties16")
