@@ -69,9 +69,9 @@ __global__ __launch_bounds__(kWG) void k_block_minloc(const float *__restrict__ 
 // (leftmost); adjacent lanes read adjacent blocks.
 __global__ __launch_bounds__(kWG) void k_block_minloc_lane(const float *__restrict__ A, int n, int bs, int nb,
                                                            Pair *__restrict__ out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const int lo = (int)((int64_t)b * bs);
+    const int lo = (int)(b * bs);
     const int hi = (int)min((int64_t)n - 1, (int64_t)lo + bs - 1);
     float bv = A[lo];
     int bi = lo;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kWG) void k_block_minloc_wave(const float *__restri
 // windows are disjoint and the left one holds the smaller indices, so the
 // left wins ties.
 __global__ void k_sparse_level(const Pair *__restrict__ prev, Pair *__restrict__ cur, int count, int half) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const Pair x = prev[i];
     const Pair y = prev[i + half];
@@ -216,8 +216,9 @@ __global__ __launch_bounds__(kWG) void k_block_sub_scan(const Pair *__restrict__
 __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pair *__restrict__ pre_s,
                               const Pair *__restrict__ suf_s, const Pair *__restrict__ excl_pre,
                               const Pair *__restrict__ excl_suf, Pair *__restrict__ pre_b, Pair *__restrict__ suf_b) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)t;
     const int b = block_of(i, magic);
     const int64_t g = (int64_t)b * nsub + ((i - b * bs) >> kSubShift);
     Pair p = excl_pre[g];
@@ -540,7 +541,7 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     for (int32_t k = 1; rc == RMQ_OK && k < levels; ++k) {
         const int half = 1 << (k - 1);
         const int count = (int)(nb - ((int64_t)1 << k) + 1);
-        k_sparse_level<<<(count + 255) / 256, 256, 0, s>>>(h->st + (int64_t)(k - 1) * nb, h->st + (int64_t)k * nb,
+        k_sparse_level<<<(unsigned)(((int64_t)count + 255) / 256), 256, 0, s>>>(h->st + (int64_t)(k - 1) * nb, h->st + (int64_t)k * nb,
                                                            count, half);
         rc = hip_rc(hipGetLastError());
     }

@@ -108,8 +108,8 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
 // Every load lies inside [a, b], so nothing past the array end is touched.
 template <int STRIDE, int U>
 __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
-    const int ah = (a + 3) & ~3;
-    const int bt = (b + 1) & ~3;
+    const int ah = (int)min(((int64_t)a + 3) & ~(int64_t)3, (int64_t)INT_MAX);  // a + 3 can pass INT_MAX
+    const int bt = (b + 1) & ~3;  // b + 1 <= INT_MAX: b < n <= INT_MAX
     const int h_end = min(ah, b + 1);
     const int t_beg = max(bt, h_end);
     if (lane < h_end - a) upd(A[a + lane], a + lane, bv, bi);

@@ -109,3 +109,20 @@ def test_gpu_edges_and_autotune(alg, tmp_path, gpu):
     assert csv.read_text().strip().split(",")[4] == m.group(1)  # the CSV row carries the picked bs
     r = run(*args)
     assert r.returncode == 0 and "; cached]" in r.stdout and "pass (0 mismatches)" in r.stdout
+
+
+@pytest.mark.gpu
+def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
+    """--save-power samples the amdgpu hwmon sensor every 100 ms during the
+    query loop into the reference's six columns (common/nvmlPower.cpp:37-84):
+    timestep, power, accumulated energy, average power, dt, accumulated time."""
+    pfile = tmp_path / "p.dat"
+    r = run(1 << 20, 1 << 16, -1, 2, "--reps", 4, "--seed", 3, f"--save-power={pfile}")
+    assert r.returncode == 0, r.stderr[-1500:]
+    if "no readable amdgpu power sensor" in r.stderr:
+        pytest.skip("no readable power sensor on this machine")
+    rows = [ln.split() for ln in pfile.read_text().splitlines() if ln.strip() and not ln.startswith("#")]
+    assert len(rows) >= 2 and all(len(x) == 6 for x in rows)
+    vals = np.array(rows, dtype=np.float64)
+    assert np.all(vals[:, 1] > 0) and np.all(np.diff(vals[:, 2]) >= 0) and np.all(np.diff(vals[:, 5]) > 0)
+    assert "power:" in r.stdout
