@@ -230,7 +230,9 @@ int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, in
 int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_ms);
 /* Latency percentiles and throughput since the last reset (reset != 0 clears). */
 int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset);
-/* Finishes the launches in flight, then frees everything (not the arrays). */
+/* Finishes the launches in flight, then frees everything (not the caller's
+ * arrays).  Requests not yet launched are dropped; no thread may submit to or
+ * wait on the server during or after this call. */
 void rmq_server_destroy(rmq_server_t *sv);
 
 /* ---- brute force on the device, algs 2/102 and the --check oracle:

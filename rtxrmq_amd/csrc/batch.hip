@@ -547,8 +547,14 @@ void scheduler_loop(rmq_server *sv) {
         free_slot->t_enqueued = now_ms();
         lk.lock();
         sv->sum_enqueue_ms += free_slot->t_enqueued - t0;
-        if (rc != RMQ_OK)
+        if (rc != RMQ_OK) {
+            // whatever was enqueued before the failure may still read the
+            // requests' host buffers: let it finish before they are released
+            lk.unlock();
+            (void)hipStreamSynchronize(free_slot->stream);
+            lk.lock();
             for (Request *r : group) r->status = rc;
+        }
         free_slot->launched = true;
         sv->cv_slot.notify_all();
     }

@@ -640,7 +640,7 @@ RMQ_API int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts
 
 RMQ_API int rmq_gpu_base(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
                          hipStream_t s) {
-    if (!d_A || n <= 0 || n > INT32_MAX || q < 0) return RMQ_ERR_INVALID_ARG;
+    if (!d_A || n <= 0 || n > INT32_MAX || q < 0 || q > INT32_MAX) return RMQ_ERR_INVALID_ARG;
     if (q == 0) return RMQ_OK;
     if (!d_Q || (!d_val && !d_idx)) return RMQ_ERR_INVALID_ARG;
     const int bsz = 256;  // the reference uses BSIZE=1024 (src/main.cu:14); wave64 prefers 256
