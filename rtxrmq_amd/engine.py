@@ -59,6 +59,17 @@ def _dev_queries(t):
     return t
 
 
+def _dev_out(t, dtype: str, q: int, device, name: str):
+    """A caller's output buffer: the kernels write q entries into it, so it
+    must be a contiguous CUDA tensor of the right dtype, on the array's device,
+    with at least q entries."""
+    torch = _torch()
+    if (not isinstance(t, torch.Tensor) or t.device != device or t.dtype != getattr(torch, dtype)
+            or not t.is_contiguous() or t.numel() < q):
+        raise TypeError(f"{name} must be a contiguous {dtype} tensor on {device} with at least {q} entries")
+    return t
+
+
 def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0, long_lanes: int = 0) -> N.QueryOpts:
     return N.QueryOpts(short_max, group_lanes, long_wgs, long_lanes)
 
@@ -98,10 +109,12 @@ class RMQIndex:
     def _outs(self, q: int, want_value: bool, want_index: bool, out_val, out_idx):
         torch = _torch()
         dev = self.A.device
-        if want_value and out_val is None:
-            out_val = torch.empty(q, dtype=torch.float32, device=dev)
-        if want_index and out_idx is None:
-            out_idx = torch.empty(q, dtype=torch.int32, device=dev)
+        if want_value:
+            out_val = (torch.empty(q, dtype=torch.float32, device=dev) if out_val is None
+                       else _dev_out(out_val, "float32", q, dev, "out_val"))
+        if want_index:
+            out_idx = (torch.empty(q, dtype=torch.int32, device=dev) if out_idx is None
+                       else _dev_out(out_idx, "int32", q, dev, "out_idx"))
         return (out_val if want_value else None), (out_idx if want_index else None)
 
     def query(self, Q, want_value: bool = True, want_index: bool = True, opts: N.QueryOpts | None = None,
@@ -256,7 +269,18 @@ class Plan:
         m = len(indexes)
         if len(Qs) != m or (out_idx is not None and len(out_idx) != m) or (out_val is not None and len(out_val) != m):
             raise ValueError("indexes, Qs and outputs must have the same length")
-        self._keep = (list(indexes), [_dev_queries(Q) for Q in Qs], out_idx, out_val)
+        Qs = [_dev_queries(Q) for Q in Qs]
+        for k, (ix, Q) in enumerate(zip(indexes, Qs)):
+            q = Q.numel() // 2
+            i_t = out_idx[k] if out_idx is not None else None
+            v_t = out_val[k] if out_val is not None else None
+            if i_t is not None:
+                _dev_out(i_t, "int32", q, ix.A.device, f"out_idx[{k}]")
+            if v_t is not None:
+                _dev_out(v_t, "float32", q, ix.A.device, f"out_val[{k}]")
+            if q and i_t is None and v_t is None:
+                raise ValueError(f"batch {k} has queries but no output")
+        self._keep = (list(indexes), Qs, out_idx, out_val)
         P = ctypes.c_void_p * m
         hs = P(*[ix.handle for ix in indexes])
         qp = P(*[Q.data_ptr() for Q in Qs])

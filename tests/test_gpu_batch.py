@@ -124,3 +124,23 @@ def test_empty_plan_and_empty_arrays(gpu):
     ev, ei = O.rmq_brute(A.cpu().numpy(), Q1.cpu().numpy())
     np.testing.assert_array_equal(out1.cpu().numpy(), ei)
     ix.close()
+
+
+def test_output_buffers_are_checked(gpu):
+    A = _dev(O.gen_array(1000, 2), gpu)
+    Q = _dev(O.gen_queries(100, 1000, -1, 3), gpu)
+    with R.RMQIndex(A, 64) as ix:
+        for bad in (torch.empty(99, dtype=torch.int32, device=gpu), torch.empty(100, dtype=torch.int64, device=gpu),
+                    torch.empty(100, dtype=torch.int32), torch.empty(200, dtype=torch.int32, device=gpu)[::2]):
+            with pytest.raises(TypeError):
+                ix.query(Q, want_value=False, out_idx=bad)
+        with pytest.raises(TypeError):
+            R.Plan([ix], [Q], out_idx=[torch.empty(50, dtype=torch.int32, device=gpu)])
+        with pytest.raises(ValueError):
+            R.Plan([ix], [Q])
+        ok = torch.full((150,), -4, dtype=torch.int32, device=gpu)  # longer than q: only q entries written
+        ix.query(Q, want_value=False, out_idx=ok)
+        torch.cuda.synchronize()
+    got = ok.cpu().numpy()
+    np.testing.assert_array_equal(got[:100], O.rmq_brute(A.cpu().numpy(), Q.cpu().numpy())[1])
+    assert np.all(got[100:] == -4)
