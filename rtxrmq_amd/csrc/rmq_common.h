@@ -251,17 +251,22 @@ __device__ __forceinline__ void answer_partial(const Blocks &k, int l, int r, in
         if (lB == rB) {
             scan_hier<G, U>(k, lB, l, r, gl, bv, bi);
         } else {
-            // The minimum of the whole blocks lB..rB covers the range; when
-            // its leftmost argmin lies inside [l, r] it is the answer (every
-            // element before it in those blocks is larger).
-            const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
-            if (sup.i >= l && sup.i <= r) {
-                mv = sup.v;
-                mi = sup.i;
-                return;
-            }
-            if (rB - lB > 1) {
-                const Pair m = table_min(k.st, k.nb, lB + 1, rB - lB - 1);
+            // The superset test (minimum of blocks lB..rB) has already been
+            // tried one lane per query by answer_class; what remains is exact
+            // without it.  The middle's two table entries and both edge
+            // blocks' row-0 entries depend only on l and r: all four reads are
+            // issued together (with no whole block between, the middle reads
+            // repeat row 0 at lB and are not used).
+            const bool has_mid = rB - lB > 1;
+            const int c = has_mid ? rB - lB - 1 : 1;
+            const int lev = 31 - __clz(c);
+            const Pair *row = k.st + (int64_t)lev * k.nb;
+            const Pair x = row[has_mid ? lB + 1 : lB];
+            const Pair y = row[has_mid ? rB - (1 << lev) : lB];
+            const Pair eb = k.st[lB];
+            const Pair er = k.st[rB];
+            if (has_mid) {
+                const Pair m = (y.v < x.v) ? y : x;
                 mv = m.v;
                 mi = m.i;
             }
@@ -270,10 +275,8 @@ __device__ __forceinline__ void answer_partial(const Blocks &k, int l, int r, in
             // the block minimum, a lower bound for the part, cannot win; only
             // otherwise scanned.  (mv, mi) is known to every lane of the group,
             // so the group takes the same branch.
-            const Pair eb = k.st[lB];
             if (eb.i >= l) take_if_better(eb.v, eb.i, mv, mi);
             else if (eb.v <= mv) scan_hier<G, U>(k, lB, l, lB * k.bs + k.bs - 1, gl, bv, bi);  // earlier: wins ties
-            const Pair er = k.st[rB];
             if (er.i <= r) take_if_better(er.v, er.i, mv, mi);
             else if (er.v < mv) scan_hier<G, U>(k, rB, rB * k.bs, r, gl, bv, bi);  // later: must be smaller
         }
