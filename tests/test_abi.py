@@ -2,6 +2,7 @@
 include/rmq.h declares, validates arguments before touching the device, and
 its host-side generators agree with the oracle's."""
 import ctypes
+import os
 import subprocess
 
 import numpy as np
@@ -10,6 +11,8 @@ import pytest
 import rtxrmq_amd as R
 from rtxrmq_amd import _native as N
 from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_loads_and_exports_header():
@@ -100,3 +103,38 @@ def test_product_path_has_no_cpu_fallback():
     pat = re.compile(r"(^\s*(from|import)\s+oracle)|liboracle|libhrmq|_ref/", re.M)
     for f in list(pkg.rglob("*.py")) + list(pkg.rglob("*.hip")) + list(pkg.rglob("*.cpp")):
         assert not pat.search(f.read_text()), f"{f} reaches into oracle/"
+
+
+@pytest.mark.parametrize("lang", ["c", "c++"])
+def test_header_compiles_and_links_for_c_and_cpp(lang, tmp_path):
+    """include/rmq.h is the boundary a C or C++ host includes (the drop-in of
+    INTEGRATION.md): it must compile on its own as C11 and C++17, and a program
+    linked against librtxrmq.so must run (calls that need no device)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc" if lang == "c" else "g++")
+    if cc is None:
+        pytest.skip("no host compiler")
+    src = tmp_path / ("t.c" if lang == "c" else "t.cpp")
+    src.write_text('''#include "rmq.h"
+#include <stdio.h>
+#include <string.h>
+int main(void) {
+    rmq_query_opts_t o = {0, 0, 0, 0};
+    rmq_server_config_t c;
+    memset(&c, 0, sizeof c);
+    (void)o;
+    if (rmq_build(0, 10, 4, 0, 0, 0) != RMQ_ERR_INVALID_ARG) return 2;
+    printf("%s|%s\\n", rmq_version(), rmq_strerror(RMQ_ERR_OOM));
+    return (int)sizeof(c) > 0 ? 0 : 3;
+}
+''')
+    lib_dir = os.path.join(N.PKG, "lib")
+    exe = tmp_path / "t"
+    std = "-std=c11" if lang == "c" else "-std=c++17"
+    r = subprocess.run([cc, std, "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe),
+                        f"-L{lib_dir}", "-lrtxrmq", f"-Wl,-rpath,{lib_dir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "gfx950" in r.stdout
