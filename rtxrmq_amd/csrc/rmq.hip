@@ -876,6 +876,9 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     } else {
         for (int b = 3; b <= 18; ++b) cands.push_back(1 << b);
     }
+    // an array smaller than every candidate is one block
+    if (std::none_of(cands.begin(), cands.end(), [&](int32_t b) { return b > 0 && b <= n; }))
+        cands.push_back((int32_t)n);
     auto base_opts = [](int32_t bs) {
         rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, bs <= 2048 ? 16 : kWave};
         return o;

@@ -124,6 +124,20 @@ def test_autotuner_pick_and_cache(tmp_path, gpu):
     assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, "tuned config")
 
 
+@pytest.mark.parametrize("n", [1, 5, 7])
+def test_autotuner_on_arrays_smaller_than_every_block_size(n, gpu):
+    A = O.gen_array(n, 3, "ties16")
+    A_dev = _dev(A, gpu)
+    Q = np.array([(l, r) for l in range(n) for r in range(l, n)], np.int32)
+    pick = R.tune(A_dev, _dev(Q, gpu))
+    assert pick["bs"] == n and not pick["from_cache"]
+    with R.RMQIndex(A_dev, pick["bs"], flags=pick["flags"]) as ix:
+        v, i = ix.query(_dev(Q, gpu), opts=pick["opts"])
+        torch.cuda.synchronize()
+    ev, ei = O.rmq_brute(A, Q)
+    assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, f"n={n}")
+
+
 def test_config1_plain_small(gpu):
     """n=2^20 U[0,1), q=2^20 uniform lengths, bs=2^15: engine vs fast oracle
     (all queries), vs CPU BASE brute force and GPU BASE (subsample), vs the
