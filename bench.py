@@ -247,7 +247,7 @@ def load_traffic(kernel: str, workload: str):
 
 
 def _counter_rows(d: str, counter: str) -> list[tuple[str, float]]:
-    """(kernel name, value in KiB) per dispatch from a rocprofv3 --pmc output tree."""
+    """(kernel name, value) per dispatch of one counter from a rocprofv3 --pmc output tree."""
     import csv
     import glob
     rows = []
@@ -279,6 +279,16 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
     for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
         for k, v in _counter_rows(os.path.join(d, sub), c):
             per.setdefault(k, {}).setdefault(c, []).append(v * 1024.0 * cal[c])
+    # optional passes: L2 (d/l2), shader sequencer (d/sq), L1 (d/tcp)
+    raw = {}
+    for sub, names in (("l2", ("TCC_HIT_sum", "TCC_MISS_sum")),
+                       ("sq", ("VALUBusy", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")),
+                       ("tcp", ("TCP_TOTAL_CACHE_ACCESSES_sum", "TCP_TCC_READ_REQ_sum", "TA_BUSY_avr",
+                                "GRBM_GUI_ACTIVE"))):
+        if os.path.isdir(os.path.join(d, sub)):
+            for c in names:
+                for k, v in _counter_rows(os.path.join(d, sub), c):
+                    raw.setdefault(k, {}).setdefault(c, []).append(v)
 
     def find(sym: str):
         # "k_query_tiles<4, false>" matches the demangled "rmqx::k_query_tiles<4, false>(...)"
@@ -301,7 +311,21 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
             if not rec or "FETCH_SIZE" not in rec or "WRITE_SIZE" not in rec:
                 continue
             fb, wb = float(np.mean(rec["FETCH_SIZE"])), float(np.mean(rec["WRITE_SIZE"]))
+            hits = [k for k in raw if f"::{sym}(" in k or k.startswith(f"{sym}(")]
+            cnt = {c: float(np.sum(v)) for c, v in raw[hits[0]].items()} if len(hits) == 1 else {}
+            extra = {}
+            if "TCC_HIT_sum" in cnt and cnt["TCC_HIT_sum"] + cnt.get("TCC_MISS_sum", 0) > 0:
+                extra["l2_hit_pct"] = 100 * cnt["TCC_HIT_sum"] / (cnt["TCC_HIT_sum"] + cnt["TCC_MISS_sum"])
+            if "VALUBusy" in cnt:
+                extra["valu_busy_pct"] = float(np.mean(raw[hits[0]]["VALUBusy"]))
+            if cnt.get("SQ_WAVE_CYCLES"):
+                extra["wave_cycles_waiting_pct"] = 100 * cnt.get("SQ_WAIT_INST_ANY", 0) / cnt["SQ_WAVE_CYCLES"]
+            if cnt.get("TCP_TOTAL_CACHE_ACCESSES_sum"):
+                extra["l1_hit_pct"] = 100 * (1 - cnt.get("TCP_TCC_READ_REQ_sum", 0) / cnt["TCP_TOTAL_CACHE_ACCESSES_sum"])
+            if cnt.get("GRBM_GUI_ACTIVE") and "TA_BUSY_avr" in cnt:
+                extra["ta_busy_pct"] = 100 * cnt["TA_BUSY_avr"] / cnt["GRBM_GUI_ACTIVE"]
             out.setdefault(workload, {})[sym] = {
+                **extra,
                 "bytes_per_launch": fb + wb, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
                 "dispatches": len(rec["FETCH_SIZE"]),
                 "algorithmic_bytes_per_launch": info["algorithmic_bytes_per_launch"],
