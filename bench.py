@@ -134,10 +134,11 @@ def left_part_min(A, sub, bs: int, l):
 
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
                       sub: BlockLevel | None = None, A=None) -> dict:
-    """Per pass: bytes in the long and the short kernel, their query counts,
+    """Per pass: bytes in the long and the short class, their query counts,
     the fraction of cross-block edge parts computed, the random gathers of
-    per-element edge pairs (with edge tables) and the sparse-table entries read.  Q: (q, 2) int32 tensor on
-    the level's device.
+    per-element edge pairs (with edge tables), the sparse-table entries read
+    and the 128-byte lines of A the short ranges span.  Q: (q, 2) int32
+    tensor on the level's device.
 
     Short class (<= short_max elements, >= 32 with edge tables): B = 8 +
     4*len + out.  Ranges inside one block without edge tables: B = 8 + the
@@ -159,7 +160,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
-    n_long = n_short = parts = needed = gathers = reads = 0
+    n_long = n_short = parts = needed = gathers = reads = lines = 0
     sm = max(short_max, 32) if edges else short_max
     for c0 in range(0, Q.shape[0], 1 << 23):
         l = Q[c0:c0 + (1 << 23), 0].long()
@@ -168,6 +169,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         is_long = ln > sm
         short_b += float((8 + 4 * ln[~is_long] + out_bytes).sum())
         n_short += int((~is_long).sum())
+        lines += int((r[~is_long] // 32 - l[~is_long] // 32 + 1).sum())
         l, r, ln = l[is_long], r[is_long], ln[is_long]
         n_long += int(l.numel())
         lB, rB = l // bs, r // bs
@@ -216,7 +218,8 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
                       + need_r.long() * scan_bytes(rB * bs, r, bs))
         long_b += float((8 + 16 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
-            "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads}
+            "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads,
+            "short_lines": lines}
 
 
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
@@ -412,7 +415,7 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
     levels: BlockLevel per block size (and 32, with edge tables)."""
     per_dist = {}
-    kernels = {}  # kernel symbol -> [algorithmic bytes, ms, launches, edge-pair gathers, table reads]
+    kernels = {}  # symbol -> [algorithmic bytes, ms, launches, edge-pair gathers, table reads, short-range lines]
     for lr in DISTS:
         e = bool(edges_for and edges_for[lr])
         if e and bs_for[lr] > 32 and bs_for[lr] % 32:
@@ -436,17 +439,23 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
             "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
         }
-        long_sym = ("k_query_edges" if edges_for and edges_for[lr]
-                    else f"k_query_tiles<{opts_for[lr].long_lanes}, true>")
-        short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
-        per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
-        for sym, b, t, g, rd in ((long_sym, lb, lms, ab["gathers"], ab["table_reads"]), (short_sym, sb, sms, 0, 0)):
-            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0, 0])
+        if e:  # one pass answers every range (k_query_edges<G>: G lanes per short range)
+            sym = f"k_query_edges<{opts_for[lr].group_lanes}>"
+            per_dist[DIST_NAMES[lr]]["kernels"] = {"all": sym}
+            launches = ((sym, lb + sb, lms, ab["gathers"], ab["table_reads"], ab["short_lines"]),)
+        else:
+            long_sym = f"k_query_tiles<{opts_for[lr].long_lanes}, true>"
+            short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
+            per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
+            launches = ((long_sym, lb, lms, 0, ab["table_reads"], 0), (short_sym, sb, sms, 0, 0, 0))
+        for sym, b, t, g, rd, ln in launches:
+            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0, 0, 0])
             acc[0] += b * len(steps_out)
             acc[1] += float(np.sum(t))
             acc[2] += len(steps_out)
             acc[3] += g * len(steps_out)
             acc[4] += rd * len(steps_out)
+            acc[5] += ln * len(steps_out)
     dom = max(kernels, key=lambda k: kernels[k][1])
     achieved = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
     per_kernel = {k: {"ms": v[1], "launches": v[2], "avg_launch_ms": v[1] / v[2],
@@ -458,6 +467,7 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             per_kernel[k]["table_reads_per_launch"] = v[4] / v[2]
             per_kernel[k]["edge_gathers_per_s"] = v[3] / (v[1] / 1e3)
             per_kernel[k]["table_reads_per_s"] = v[4] / (v[1] / 1e3)
+            per_kernel[k]["short_lines_per_launch"] = v[5] / v[2]
     return {"value": 3 * q * len(steps_out) / elapsed, "ms_per_step": elapsed / len(steps_out) * 1e3,
             "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
 
@@ -792,13 +802,16 @@ def main() -> int:
     if "table_reads_per_s" in dk and not args.no_gather_probe:
         l2 = R.probe_gather(table_bytes=1 << 21, device=dev)["gathers_per_s"]
         dram = R.probe_gather(table_bytes=1 << 30, device=dev)["gathers_per_s"]
-        t_bound = dk["table_reads_per_launch"] / l2 + dk["edge_gathers_per_launch"] / dram
+        t_bound = (dk["table_reads_per_launch"] / l2
+                   + (dk["edge_gathers_per_launch"] + dk["short_lines_per_launch"]) / dram)
         gather_ceiling = {
             "l2_table_reads_per_s": l2, "dram_gathers_per_s": dram,
             "kernel_table_reads_per_s": dk["table_reads_per_s"], "kernel_edge_gathers_per_s": dk["edge_gathers_per_s"],
+            "kernel_short_lines_per_s": dk["short_lines_per_launch"] / (dk["avg_launch_ms"] / 1e3),
             "frac": t_bound / (dk["avg_launch_ms"] / 1e3),
             "probe": "rmq_probe_gather: 2^26 threads x 2 random 8 B reads from two 2 MiB tables (L2) and from two "
-                     "1 GiB tables (DRAM); frac = (table reads / L2 rate + edge-pair gathers / DRAM rate) / time"}
+                     "1 GiB tables (DRAM); frac = (table reads / L2 rate + (edge-pair gathers + 128 B lines of "
+                     "short ranges) / DRAM rate) / time"}
         log(f"bench: random-read ceilings: L2 tables {l2:.3e}/s, DRAM {dram:.3e}/s; {tuned['dominant']} "
             f"{dk['table_reads_per_s']:.3e} table reads/s + {dk['edge_gathers_per_s']:.3e} edge gathers/s "
             f"= {gather_ceiling['frac']:.3f} of those rates")

@@ -65,8 +65,10 @@ typedef struct rmq_query_opts {
 
 /* Per-kernel device time of one rmq_query_timed call, milliseconds. */
 typedef struct rmq_query_times {
-    float short_ms; /* binning pass fused with the short-range group scans */
-    float long_ms;  /* long-range wave scans (edge blocks + table)          */
+    float short_ms; /* binning pass fused with the short-range group scans;
+                       0 with edge tables (one pass answers every range)   */
+    float long_ms;  /* long-range wave scans (edge blocks + table); with
+                       edge tables the single pass over all queries        */
     float total_ms; /* from the first launch to the last                   */
 } rmq_query_times_t;
 

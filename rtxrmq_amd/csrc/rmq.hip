@@ -231,21 +231,27 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
     suf_b[i] = sf;
 }
 
-// Ranges of more than short_max (>= 32) elements with edge tables: one lane
-// per query, four O(1) lookups (answer_edges), no scan and no reduction.
+// Every query of a handle with edge tables, in one pass over the query
+// array: each wave loads a tile of 64 queries once, answers the ranges of
+// more than short_max (>= 32) elements one lane per query by O(1) lookups
+// (answer_edges: no scan, no reduction), then the short ranges with G-lane
+// group scans (answer_class), and writes (NaN, -1) for invalid queries.
+template <int G>
 __global__ __launch_bounds__(kWG) void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
                                                      int short_max, float *__restrict__ out_v,
                                                      int32_t *__restrict__ out_i) {
+    __shared__ TileStage stage[kWavesPerWG];
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, false, out_v, out_i);
+        const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, true, out_v, out_i);
         if ((m.longs >> lane) & 1) {
             const Pair best = answer_edges(e, k, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
+        if (m.shorts) answer_class<G, false>(k, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
     }
 }
 
@@ -372,8 +378,11 @@ int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int g
 
 int effective_short_max(const rmq_handle *h, const rmq_query_opts_t &o);
 
+// With edge tables the long pass answers the short ranges too (k_query_edges),
+// so there is no separate short pass.
 int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                  hipStream_t s) {
+    if (h->edges.pre_b) return RMQ_OK;
     return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), effective_short_max(h, o), val, idx, s);
 }
 
@@ -388,8 +397,14 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
     int grid = tiles_grid(q, 8);
     if (o.long_wgs > 0) grid = (int)std::min<int64_t>(o.long_wgs, grid);
     if (h->edges.pre_b) {
-        k_query_edges<<<grid, kWG, 0, s>>>(blocks_of(h), h->edges, Q, q,
-                                           effective_short_max(h, o), val, idx);
+        const Blocks k = blocks_of(h);
+        const int sm = effective_short_max(h, o);
+        switch (o.group_lanes) {
+        case 4: k_query_edges<4><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+        case 8: k_query_edges<8><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+        case 16: k_query_edges<16><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+        default: k_query_edges<64><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+        }
         return hip_rc(hipGetLastError());
     }
     return launch_tiles<true>(h, Q, q, o.long_lanes, grid, o.short_max, val, idx, s);
