@@ -140,7 +140,8 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     and the 128-byte lines of A the short ranges span.  Q: (q, 2) int32
     tensor on the level's device.
 
-    Short class (<= short_max elements, >= 32 with edge tables): B = 8 +
+    Short class (<= short_max elements; with edge tables also every range
+    inside one 32-element sub-block, one block when bs <= 32): B = 8 +
     4*len + out.  Ranges inside one block without edge tables: B = 8 + the
     scan's bytes (scan_bytes: 4 per element, or 8 per whole 32-element
     sub-block between the ends for long ones) + out.  Every other
@@ -161,12 +162,14 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     bs = lvl.bs
     long_b = short_b = 0.0
     n_long = n_short = parts = needed = gathers = reads = lines = 0
-    sm = max(short_max, 32) if edges else short_max
     for c0 in range(0, Q.shape[0], 1 << 23):
         l = Q[c0:c0 + (1 << 23), 0].long()
         r = Q[c0:c0 + (1 << 23), 1].long()
         ln = r - l + 1
-        is_long = ln > sm
+        is_long = ln > short_max
+        if edges:  # inside one sub-block: scanned whatever short_max is
+            base = l // bs * bs
+            is_long &= ~((r // bs * bs == base) & ((l - base) // 32 == (r - base) // 32))
         short_b += float((8 + 4 * ln[~is_long] + out_bytes).sum())
         n_short += int((~is_long).sum())
         lines += int((r[~is_long] // 32 - l[~is_long] // 32 + 1).sum())

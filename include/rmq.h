@@ -57,7 +57,9 @@ typedef struct rmq_handle rmq_handle_t;
 
 /* Query tuning; zero fields take the defaults (rmq_query_opts_default). */
 typedef struct rmq_query_opts {
-    int32_t short_max;   /* ranges of at most this many elements go to the group scan   */
+    int32_t short_max;   /* ranges of at most this many elements go to the group scan;
+                            with edge tables so does every range inside one 32-element
+                            sub-block (one block when bs <= 32), all others are O(1)  */
     int32_t group_lanes; /* lanes per short query: 4, 8, 16 or 64                       */
     int32_t long_wgs;    /* persistent workgroups for the long-range kernel (0 = auto)  */
     int32_t long_lanes;  /* lanes per long query: 4, 8, 16 or 64                        */
@@ -82,8 +84,9 @@ typedef struct rmq_query_times {
  *      flags: 0, or RMQ_BUILD_EDGES to also precompute, per element, the
  *      leftmost (min, argmin) from its block start and to its block end, and
  *      the same inside 32-element sub-blocks with a sparse table over them
- *      (32 bytes per element): a range of more than 32 elements is then
- *      answered by one lane with four O(1) lookups instead of edge scans.
+ *      (32 bytes per element): every range not inside one 32-element
+ *      sub-block (one block when bs <= 32) is then answered by one lane with
+ *      a few O(1) lookups instead of edge scans.
  *      Stream-ordered; the handle is usable on stream s at once, on other
  *      streams after s is synchronised. ---- */
 int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out);

@@ -62,10 +62,10 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
         const WorkItem w = items[it];
         const ArrayDesc d = descs[w.array];
         const bool edges = d.edges.pre_b != nullptr;  // uniform across the wave
-        const int sm = edges ? max(short_max, kSub) : short_max;
         const int64_t tiles = ((int64_t)w.count + kWave - 1) / kWave;
         for (int64_t t = 0; t < tiles; ++t) {
-            const TileMasks m = load_tile(w.Q, w.count, d.k.n, sm, t, lane, true, w.out_v, w.out_i);
+            const TileMasks m = edges ? load_tile_edges(w.Q, w.count, d.k, short_max, t, lane, w.out_v, w.out_i)
+                                      : load_tile(w.Q, w.count, d.k.n, short_max, t, lane, true, w.out_v, w.out_i);
             answer_class<GS, false>(d.k, m.shorts, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
             if (edges) {
                 if ((m.longs >> lane) & 1) {

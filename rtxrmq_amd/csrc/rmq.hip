@@ -232,8 +232,8 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
 }
 
 // Every query of a handle with edge tables, in one pass over the query
-// array: each wave loads a tile of 64 queries once, answers the ranges of
-// more than short_max (>= 32) elements one lane per query by O(1) lookups
+// array: each wave loads a tile of 64 queries once, answers the ranges that
+// are not short (load_tile_edges) one lane per query by O(1) lookups
 // (answer_edges: no scan, no reduction), then the short ranges with G-lane
 // group scans (answer_class), and writes (NaN, -1) for invalid queries.
 template <int G>
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kWG) void k_query_edges(Blocks k, EdgeTables e, con
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, true, out_v, out_i);
+        const TileMasks m = load_tile_edges(Q, q, k, short_max, t, lane, out_v, out_i);
         if ((m.longs >> lane) & 1) {
             const Pair best = answer_edges(e, k, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
@@ -376,20 +376,12 @@ int launch_tiles(const rmq_handle *h, const int2 *Q, int64_t q, int lanes, int g
     return hip_rc(hipGetLastError());
 }
 
-int effective_short_max(const rmq_handle *h, const rmq_query_opts_t &o);
-
 // With edge tables the long pass answers the short ranges too (k_query_edges),
 // so there is no separate short pass.
 int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                  hipStream_t s) {
     if (h->edges.pre_b) return RMQ_OK;
-    return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), effective_short_max(h, o), val, idx, s);
-}
-
-// With edge tables every range of more than 32 elements is O(1), so the
-// short class is widened to at least 32 elements.
-int effective_short_max(const rmq_handle *h, const rmq_query_opts_t &o) {
-    return h->edges.pre_b ? std::max(o.short_max, kSub) : o.short_max;
+    return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), o.short_max, val, idx, s);
 }
 
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
@@ -398,7 +390,7 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
     if (o.long_wgs > 0) grid = (int)std::min<int64_t>(o.long_wgs, grid);
     if (h->edges.pre_b) {
         const Blocks k = blocks_of(h);
-        const int sm = effective_short_max(h, o);
+        const int sm = o.short_max;
         switch (o.group_lanes) {
         case 4: k_query_edges<4><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
         case 8: k_query_edges<8><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
@@ -903,7 +895,8 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     int tried = 0;
     // stage 1: block size, with and without edge tables, default lanes; each
     // at a small and at the default short-range threshold (with the default
-    // alone, short ranges would all be scanned and hide the block size)
+    // alone, short ranges would all be scanned and hide the block size), and
+    // with edge tables also at 1 (only ranges inside one sub-block scanned)
     struct Cand {
         float ms;
         int32_t bs;
@@ -918,7 +911,8 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             if (rc == RMQ_ERR_OOM && flags) continue;  // no room for edge tables: plain only
             if (rc != RMQ_OK) return rc;
             float ms = 1e30f;
-            for (int32_t sm : {32, kDefaultShortMax}) {
+            for (int32_t sm : {1, 32, kDefaultShortMax}) {
+                if (sm == 1 && !flags) continue;
                 rmq_query_opts_t o = base_opts(bs);
                 o.short_max = sm;
                 float t = 0.f;
@@ -947,7 +941,8 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         rmq_query_opts_t cand_o = base_opts(c.bs);
         float cand_ms = 1e30f;
         if ((rc = time_pass(h, d_Q, q, d_val, d_idx, cand_o, s, kReps, &cand_ms)) == RMQ_OK) ++tried;
-        for (int32_t sm : {32, 128, 512, 1024, 4096}) {
+        for (int32_t sm : {1, 8, 32, 128, 512, 1024, 4096}) {
+            if (sm < 32 && !c.flags) continue;
             for (int32_t g : {4, 8, 16}) {
                 if (rc != RMQ_OK) break;
                 rmq_query_opts_t o = cand_o;

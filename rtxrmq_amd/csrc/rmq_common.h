@@ -196,6 +196,32 @@ struct Blocks {
     int n, bs, nb, nsub;
 };
 
+// With edge tables every range is answered in O(1) (answer_edges) except
+// one inside a single 32-element sub-block (a single block when bs <= 32):
+// those, and ranges of at most short_max elements, go to the group scan.
+// Otherwise as load_tile, with invalid queries flagged.
+__device__ __forceinline__ TileMasks load_tile_edges(const int2 *__restrict__ Q, int64_t q, const Blocks &k,
+                                                     int short_max, int64_t t, int lane, float *out_v,
+                                                     int32_t *out_i) {
+    const int64_t qi = t * kWave + lane;
+    const bool in = qi < q;
+    int l = 0, r = -1;
+    if (in) {
+        const int2 p = Q[qi];
+        l = p.x;
+        r = p.y;
+    }
+    const bool ok = in && l >= 0 && l <= r && r < k.n;
+    if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    bool is_short = ok && (r - l) < short_max;
+    if (ok && !is_short && r - l < kSub) {
+        const int bl = block_of(l, k.magic);
+        const int base = bl * k.bs;  // <= l
+        is_short = block_of(r, k.magic) == bl && ((l - base) >> kSubShift) == ((r - base) >> kSubShift);
+    }
+    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+}
+
 // Scans pairs P[a..b] (each a (min, leftmost argmin) of consecutive elements,
 // in index order) with STRIDE lanes, U loads per lane in flight.
 template <int STRIDE, int U>
@@ -299,11 +325,11 @@ struct EdgeTables {
 };
 
 
-// O(1) answer of a range of more than 32 elements from the edge tables:
-// blocks differ -> suffix of the left block, whole blocks between (block
-// table), prefix of the right block; same block -> the same three parts one
-// level down (sub-blocks of the block; a range of > 32 elements spans at
-// least two).  Parts are combined left to right with strict '<', so the
+// O(1) answer, from the edge tables, of a range not inside one 32-element
+// sub-block: blocks differ -> suffix of the left block, whole blocks between
+// (block table), prefix of the right block; same block -> the same three
+// parts one level down (the range spans at least two sub-blocks of the
+// block).  Parts are combined left to right with strict '<', so the
 // leftmost minimum wins.  One lane per query: no reduction needed.
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
     const Pair *__restrict__ st = k.st;

@@ -106,20 +106,23 @@ def test_tuning_knobs_do_not_change_answers(group_lanes, long_lanes, short_max, 
         assert_exact(v, i, ev, ei, f"bs={bs} flags={flags} G={group_lanes} GL={long_lanes} short_max={short_max}")
 
 
+@pytest.mark.parametrize("short_max", [1, 64])
 @pytest.mark.parametrize("group_lanes", [4, 8, 16, 64])
-def test_short_ranges_at_every_alignment_and_the_array_end(group_lanes, gpu):
+def test_short_ranges_at_every_alignment_and_the_array_end(group_lanes, short_max, gpu):
     """Every range of at most 40 elements of arrays whose length is and is not
     a multiple of 4: short ranges start and end at each position of a 16-byte
-    chunk, and the last ones end in the array's partial final chunk."""
-    opts = R.query_opts(short_max=64, group_lanes=group_lanes, long_lanes=16)
+    chunk, and the last ones end in the array's partial final chunk.  With
+    edge tables and short_max 1, only ranges inside one 32-element sub-block
+    (one block when bs <= 32) are scanned; the rest take the O(1) path."""
+    opts = R.query_opts(short_max=short_max, group_lanes=group_lanes, long_lanes=16)
     for n in (1, 3, 6, 61, 62, 63, 64, 1001, 1002, 1003, 4096):
         A = np.random.default_rng(n).integers(0, 5, n).astype(np.float32)
         A[::7] = np.inf
         Q = np.array([(l, r) for l in range(n) for r in range(l, min(n, l + 40))], np.int32)
         ev, ei = O.rmq_brute(A, Q)
-        for bs, flags in ((8, 0), (8, R.BUILD_EDGES), (64, R.BUILD_EDGES), (1000, 0)):
+        for bs, flags in ((8, 0), (8, R.BUILD_EDGES), (64, R.BUILD_EDGES), (100, R.BUILD_EDGES), (1000, 0)):
             v, i = run_engine(A, Q, bs, gpu, opts=opts, flags=flags)
-            assert_exact(v, i, ev, ei, f"n={n} bs={bs} flags={flags} G={group_lanes}")
+            assert_exact(v, i, ev, ei, f"n={n} bs={bs} flags={flags} G={group_lanes} short_max={short_max}")
 
 
 def test_autotuner_pick_and_cache(tmp_path, gpu):

@@ -126,9 +126,12 @@ def test_superset_rule_inside_one_block_over_sub_blocks():
                         assert sup == brute(A, l, r), (bs, l, r)
 
 
-@pytest.mark.parametrize("edges", [False, True])
-def test_byte_model_replays_the_rules(edges):
-    rng = np.random.default_rng(11 + edges)
+@pytest.mark.parametrize("edges,short_max", [(False, 32), (True, 32), (True, 1)])
+def test_byte_model_replays_the_rules(edges, short_max):
+    """With edge tables a range inside one 32-element sub-block (one block
+    when bs <= 32) is scanned whatever short_max is; every other range longer
+    than short_max takes the O(1) path."""
+    rng = np.random.default_rng(11 + edges + short_max)
     for trial in range(16):
         n = int(rng.integers(100, 2500))
         bs = int(rng.choice([8, 24, 64, 128, 256] if edges else [4, 16, 37, 64, 256]))
@@ -142,11 +145,13 @@ def test_byte_model_replays_the_rules(edges):
             Q.append((l, r))
         Q = np.array(Q, np.int32)
         bm = block_mins(A, bs)
-        parts = need = gathers = 0
+        parts = need = gathers = n_short = 0
         for l, r in Q:
-            if r - l + 1 <= 32:
-                continue
             lB, rB = l // bs, r // bs
+            one_sub = lB == rB and (l - lB * bs) // 32 == (r - lB * bs) // 32
+            if r - l + 1 <= short_max or (edges and one_sub):
+                n_short += 1
+                continue
             if lB == rB:
                 if edges:
                     base = lB * bs
@@ -161,7 +166,9 @@ def test_byte_model_replays_the_rules(edges):
             gathers += k if edges else 0
         lvl = bench.BlockLevel(torch.from_numpy(A), bs)
         sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
-        ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, 32, edges=edges, sub=sub, A=torch.from_numpy(A))
+        ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, short_max, edges=edges, sub=sub,
+                                     A=torch.from_numpy(A))
+        assert ab["n_short"] == n_short
         assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
         assert ab["gathers"] == gathers
         for k in range(lvl.nb):  # block minima: leftmost argmin, all-+inf block -> its start
