@@ -414,9 +414,10 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
     return steps_out, time.perf_counter() - t_start
 
 
-def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None) -> dict:
+def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None, kernels_for=None) -> dict:
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
-    levels: BlockLevel per block size (and 32, with edge tables)."""
+    levels: BlockLevel per block size (and 32, with edge tables); kernels_for:
+    the kernel names each batch's handle launches (RMQIndex.kernels)."""
     per_dist = {}
     kernels = {}  # symbol -> [algorithmic bytes, ms, launches, edge-pair gathers, table reads, short-range lines]
     for lr in DISTS:
@@ -442,8 +443,8 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
             "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
         }
-        if e:  # one pass answers every range (k_query_edges<G>: G lanes per short range)
-            sym = f"k_query_edges<{opts_for[lr].group_lanes}>"
+        if e:  # one pass answers every range (G lanes per scanned range)
+            sym = kernels_for[lr][0] if kernels_for else f"k_query_edges<{opts_for[lr].group_lanes}>"
             per_dist[DIST_NAMES[lr]]["kernels"] = {"all": sym}
             launches = ((sym, lb + sb, lms, ab["gathers"], ab["table_reads"], ab["short_lines"]),)
         else:
@@ -465,6 +466,8 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
                       "algorithmic_bytes_per_launch": v[0] / v[2],
                       "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
     for k, v in kernels.items():
+        if k.startswith("k_query_edges_lds"):
+            per_kernel[k]["table_in_lds"] = True
         if v[3] or v[4]:
             per_kernel[k]["edge_gathers_per_launch"] = v[3] / v[2]
             per_kernel[k]["table_reads_per_launch"] = v[4] / v[2]
@@ -794,8 +797,9 @@ def main() -> int:
                                    "tuned")
     for b in (set(bs_for.values()) | ({32} if any(p["flags"] for p in picks.values()) else set())) - set(levels):
         levels[b] = BlockLevel(A, b)
+    kernels_for = {lr: index_for[lr].kernels(opts_for[lr]) for lr in DISTS}
     tuned = summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
-                      {lr: picks[lr]["flags"] for lr in DISTS})
+                      {lr: picks[lr]["flags"] for lr in DISTS}, kernels_for)
     log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
     for ix in indexes.values():
@@ -805,7 +809,7 @@ def main() -> int:
     if "table_reads_per_s" in dk and not args.no_gather_probe:
         l2 = R.probe_gather(table_bytes=1 << 21, device=dev)["gathers_per_s"]
         dram = R.probe_gather(table_bytes=1 << 30, device=dev)["gathers_per_s"]
-        t_bound = (dk["table_reads_per_launch"] / l2
+        t_bound = ((0.0 if dk.get("table_in_lds") else dk["table_reads_per_launch"] / l2)
                    + (dk["edge_gathers_per_launch"] + dk["short_lines_per_launch"]) / dram)
         gather_ceiling = {
             "l2_table_reads_per_s": l2, "dram_gathers_per_s": dram,

@@ -255,6 +255,37 @@ __global__ __launch_bounds__(kWG) void k_query_edges(Blocks k, EdgeTables e, con
     }
 }
 
+// The same pass with the block sparse table copied into LDS first, for a
+// table that fits: the superset test and the edge blocks' (min, argmin) then
+// read LDS, and the vector-memory path (whose address and L1 stages bound
+// the pass above: two divergent table reads per query) carries only the
+// queries, the results, edge pairs and scans.  Workgroups of kLdsWaves waves,
+// two resident per CU.
+template <int G>
+__global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, EdgeTables e, int table_pairs,
+                                                                     const int2 *__restrict__ Q, int64_t q,
+                                                                     int short_max, float *__restrict__ out_v,
+                                                                     int32_t *__restrict__ out_i) {
+    extern __shared__ Pair lds_table[];
+    __shared__ TileStage stage[kLdsWaves];
+    for (int j = threadIdx.x; j < table_pairs; j += kLdsWaves * kWave) lds_table[j] = k.st[j];
+    __syncthreads();
+    Blocks kl = k;
+    kl.st = lds_table;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kLdsWaves;
+    for (int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv; t < n_tiles; t += wave_stride) {
+        const TileMasks m = load_tile_edges(Q, q, kl, short_max, t, lane, out_v, out_i);
+        if ((m.longs >> lane) & 1) {
+            const Pair best = answer_edges(e, kl, m.l, m.r);
+            write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
+        }
+        if (m.shorts) answer_class<G, false>(kl, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
+    }
+}
+
 // Tile kernel: each wave takes tiles of 64 consecutive queries and answers
 // the ones of its class (LONG = more than short_max elements) with
 // answer_class.  The short-class kernel also writes (NaN, -1) for invalid
@@ -384,6 +415,33 @@ int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_
     return launch_tiles<false>(h, Q, q, o.group_lanes, tiles_grid(q, 16), o.short_max, val, idx, s);
 }
 
+// Whether one workgroup of k_query_edges_lds with `bytes` of LDS fits twice
+// in a CU (full occupancy) and within the per-workgroup limit.
+bool lds_table_fits(size_t bytes) {
+    static int per_block = -1, per_cu = -1;
+    if (per_block < 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) {
+            per_block = per_cu = 0;
+        }
+    }
+    return bytes <= (size_t)per_block && 2 * bytes <= (size_t)per_cu;
+}
+
+// Which kernel answers a handle's long ranges (and, with edge tables, all of
+// its ranges in one pass).
+enum class LongPass { kTiles, kEdges, kEdgesLds };
+
+size_t lds_table_bytes(const rmq_handle *h) { return (size_t)h->levels * (size_t)h->nb * sizeof(Pair); }
+
+LongPass long_pass(const rmq_handle *h) {
+    if (!h->edges.pre_b) return LongPass::kTiles;
+    return lds_table_fits(lds_table_bytes(h) + sizeof(TileStage) * kLdsWaves) ? LongPass::kEdgesLds
+                                                                                : LongPass::kEdges;
+}
+
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                 hipStream_t s) {
     int grid = tiles_grid(q, 8);
@@ -391,6 +449,21 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
     if (h->edges.pre_b) {
         const Blocks k = blocks_of(h);
         const int sm = o.short_max;
+        if (long_pass(h) == LongPass::kEdgesLds) {
+            const size_t lds = lds_table_bytes(h);
+            const int64_t tiles = (q + kWave - 1) / kWave;
+            int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + kLdsWaves - 1) / kLdsWaves,
+                                                                 (int64_t)device_cus() * 2));
+            if (o.long_wgs > 0) g2 = std::min(g2, o.long_wgs);
+            const int p = (int)(lds / sizeof(Pair));
+            switch (o.group_lanes) {
+            case 4: k_query_edges_lds<4><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
+            case 8: k_query_edges_lds<8><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
+            case 16: k_query_edges_lds<16><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
+            default: k_query_edges_lds<64><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
+            }
+            return hip_rc(hipGetLastError());
+        }
         switch (o.group_lanes) {
         case 4: k_query_edges<4><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
         case 8: k_query_edges<8><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
@@ -637,6 +710,24 @@ RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int6
     if (levels) *levels = h->levels;
     if (bytes) *bytes = h->st_bytes + h->edge_bytes + h->sub_bytes;
     return RMQ_OK;
+}
+
+RMQ_API int rmq_query_kernels(const rmq_handle_t *h, const rmq_query_opts_t *opts, char *buf, int64_t len) {
+    if (!h || !buf || len <= 0) return RMQ_ERR_INVALID_ARG;
+    rmq_query_opts_t o;
+    int rc = resolve_opts(h, opts, &o);
+    if (rc != RMQ_OK) return rc;
+    if ((rc = check_device(h)) != RMQ_OK) return rc;
+    int w = 0;
+    switch (long_pass(h)) {
+    case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d>", o.group_lanes); break;
+    case LongPass::kEdges: w = snprintf(buf, (size_t)len, "k_query_edges<%d>", o.group_lanes); break;
+    default:
+        w = snprintf(buf, (size_t)len, "k_query_tiles<%d, false> k_query_tiles<%d, true>", o.group_lanes,
+                     o.long_lanes);
+        break;
+    }
+    return (w < 0 || w >= len) ? RMQ_ERR_INVALID_ARG : RMQ_OK;
 }
 
 RMQ_API int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts) {

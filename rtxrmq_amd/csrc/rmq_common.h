@@ -18,6 +18,7 @@ namespace rmqx {
 constexpr int kWave = 64;
 constexpr int kWG = 256;                 // threads per workgroup (4 waves)
 constexpr int kWavesPerWG = kWG / kWave;
+constexpr int kLdsWaves = 16;            // waves per workgroup of the LDS-table query kernel
 constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
 constexpr int kDefaultShortMax = 1024;
 constexpr int kDefaultGroupLanes = 16;

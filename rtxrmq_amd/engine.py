@@ -101,6 +101,13 @@ class RMQIndex:
                                         ctypes.byref(lev), ctypes.byref(byt)), "rmq_handle_info")
         return {"n": n.value, "bs": bs.value, "nb": nb.value, "levels": lev.value, "bytes": byt.value}
 
+    def kernels(self, opts: N.QueryOpts | None = None) -> list[str]:
+        """The kernels a query on this handle launches, as a profiler names them."""
+        buf = ctypes.create_string_buffer(256)
+        N.check(N.lib().rmq_query_kernels(self._h, ctypes.byref(opts) if opts is not None else None, buf, 256),
+                "rmq_query_kernels")
+        return [k + ">" if not k.endswith(">") else k for k in buf.value.decode().split("> ") if k]
+
     def default_opts(self) -> N.QueryOpts:
         o = N.QueryOpts()
         N.check(N.lib().rmq_query_opts_default(self._h, ctypes.byref(o)), "rmq_query_opts_default")
