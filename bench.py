@@ -225,6 +225,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             "short_lines": lines}
 
 
+XCDS = 8  # accelerator complex dies of an MI355X (GRBM_GUI_ACTIVE counts each)
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -288,7 +289,7 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
     # optional passes: L2 (d/l2), shader sequencer (d/sq), L1 (d/tcp)
     raw = {}
     for sub, names in (("l2", ("TCC_HIT_sum", "TCC_MISS_sum")),
-                       ("sq", ("VALUBusy", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")),
+                       ("sq", ("VALUBusy", "SQ_WAIT_INST_ANY", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES")),
                        ("tcp", ("TCP_TOTAL_CACHE_ACCESSES_sum", "TCP_TCC_READ_REQ_sum", "TA_BUSY_avr",
                                 "GRBM_GUI_ACTIVE"))):
         if os.path.isdir(os.path.join(d, sub)):
@@ -326,10 +327,13 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
                 extra["valu_busy_pct"] = float(np.mean(raw[hits[0]]["VALUBusy"]))
             if cnt.get("SQ_WAVE_CYCLES"):
                 extra["wave_cycles_waiting_pct"] = 100 * cnt.get("SQ_WAIT_INST_ANY", 0) / cnt["SQ_WAVE_CYCLES"]
+                if "SQ_WAIT_ANY" in cnt:  # s_waitcnt: waiting for memory
+                    extra["wave_cycles_waitcnt_pct"] = 100 * cnt["SQ_WAIT_ANY"] / cnt["SQ_WAVE_CYCLES"]
             if cnt.get("TCP_TOTAL_CACHE_ACCESSES_sum"):
                 extra["l1_hit_pct"] = 100 * (1 - cnt.get("TCP_TCC_READ_REQ_sum", 0) / cnt["TCP_TOTAL_CACHE_ACCESSES_sum"])
             if cnt.get("GRBM_GUI_ACTIVE") and "TA_BUSY_avr" in cnt:
-                extra["ta_busy_pct"] = 100 * cnt["TA_BUSY_avr"] / cnt["GRBM_GUI_ACTIVE"]
+                # GRBM_GUI_ACTIVE sums the GPU's XCDs; TA_BUSY_avr is one TA's busy cycles
+                extra["ta_busy_pct"] = 100 * cnt["TA_BUSY_avr"] / (cnt["GRBM_GUI_ACTIVE"] / XCDS)
             out.setdefault(workload, {})[sym] = {
                 **extra,
                 "bytes_per_launch": fb + wb, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
@@ -834,6 +838,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds)
 
+    traffic = load_traffic(tuned["dominant"], workload)
+    dom_ms = tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"]
     line = {
         "metric": "RMQ/s at n=1e8,q=2^26 per length dist, %HBM roofline; batched/served vs plain",
         "value": tuned["value"],
@@ -857,7 +863,11 @@ def main() -> int:
                    "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},
         "roofline": {"bound": "hbm", "achieved": tuned["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tuned["achieved"] / HBM_PEAK_GBS,
-                     "traffic": load_traffic(tuned["dominant"], workload), "kernel": tuned["dominant"],
+                     "traffic": traffic, "kernel": tuned["dominant"],
+                     # the HBM bytes the counters saw, over the same launch time: random
+                     # 8-byte reads each move a whole 128-byte DRAM request
+                     "traffic_GBps": traffic / (dom_ms / 1e3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (dom_ms / 1e3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "avg_launch_ms": tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"],
                      "algorithmic_bytes_per_launch":
                          tuned["per_kernel"][tuned["dominant"]]["algorithmic_bytes_per_launch"],
