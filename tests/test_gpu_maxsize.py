@@ -4,7 +4,8 @@ Every index computation near the top of the int32 range (block starts, the
 16-byte alignment of scans, thread ids of per-element build kernels, grid
 sizes) is exercised by three structurally different indexes over the same
 8.6 GB array -- bs=2^15 without edge tables, bs=1000 with them, bs=7 (a 29-
-level table) -- which must agree bit for bit.  Every answer is checked to lie
+level table), bs=2^24 with edge tables (128 blocks: the block sparse table is
+read from LDS) -- which must agree bit for bit.  Every answer is checked to lie
 in [l, r] and to be the value stored there; ranges up to 4096 elements are
 checked against the brute force on the device (algs 2/102), and long ranges
 (including [0, n-1]) against a torch minimum and first match.
@@ -40,8 +41,10 @@ def test_maximum_array_size(gpu):
     Q = queries(np.random.default_rng(5))
     Qd = torch.from_numpy(Q).to(gpu)
     results = []
-    for bs, flags in ((1 << 15, 0), (1000, R.BUILD_EDGES), (7, 0)):
+    for bs, flags in ((1 << 15, 0), (1000, R.BUILD_EDGES), (7, 0), (1 << 24, R.BUILD_EDGES)):
         with R.RMQIndex(A, bs, flags=flags) as ix:
+            if bs == 1 << 24:
+                assert ix.kernels()[0].startswith("k_query_edges_lds<")
             v, i = ix.query(Qd)
             torch.cuda.synchronize()
             results.append((bs, flags, v.cpu().numpy(), i.cpu().numpy()))
