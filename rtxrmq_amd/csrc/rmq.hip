@@ -276,8 +276,14 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t n_tiles = (q + kWave - 1) / kWave;
     const int64_t wave_stride = (int64_t)gridDim.x * kLdsWaves;
-    for (int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv; t < n_tiles; t += wave_stride) {
-        const TileMasks m = load_tile_edges(Q, q, kl, short_max, t, lane, out_v, out_i);
+    // The next tile's queries are loaded while this tile is answered: with the
+    // table in LDS, the wait on them is most of a tile's latency.
+    int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv;
+    int2 cur = fetch_query(Q, q, t, lane);
+    for (; t < n_tiles; t += wave_stride) {
+        const int2 nxt = fetch_query(Q, q, t + wave_stride, lane);
+        const TileMasks m = classify_edges(cur, q, kl, short_max, t, lane, out_v, out_i);
+        cur = nxt;
         if ((m.longs >> lane) & 1) {
             const Pair best = answer_edges(e, kl, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);

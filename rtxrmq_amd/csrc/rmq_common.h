@@ -201,17 +201,18 @@ struct Blocks {
 // one inside a single 32-element sub-block (a single block when bs <= 32):
 // those, and ranges of at most short_max elements, go to the group scan.
 // Otherwise as load_tile, with invalid queries flagged.
-__device__ __forceinline__ TileMasks load_tile_edges(const int2 *__restrict__ Q, int64_t q, const Blocks &k,
-                                                     int short_max, int64_t t, int lane, float *out_v,
-                                                     int32_t *out_i) {
+// Query t*64+lane, or the empty range (0, -1) past the end.
+__device__ __forceinline__ int2 fetch_query(const int2 *__restrict__ Q, int64_t q, int64_t t, int lane) {
+    const int64_t qi = t * kWave + lane;
+    return qi < q ? Q[qi] : make_int2(0, -1);
+}
+
+// load_tile_edges on a query already loaded (p = fetch_query(Q, q, t, lane)).
+__device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blocks &k, int short_max, int64_t t,
+                                                    int lane, float *out_v, int32_t *out_i) {
     const int64_t qi = t * kWave + lane;
     const bool in = qi < q;
-    int l = 0, r = -1;
-    if (in) {
-        const int2 p = Q[qi];
-        l = p.x;
-        r = p.y;
-    }
+    const int l = p.x, r = p.y;
     const bool ok = in && l >= 0 && l <= r && r < k.n;
     if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     bool is_short = ok && (r - l) < short_max;
@@ -221,6 +222,12 @@ __device__ __forceinline__ TileMasks load_tile_edges(const int2 *__restrict__ Q,
         is_short = block_of(r, k.magic) == bl && ((l - base) >> kSubShift) == ((r - base) >> kSubShift);
     }
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+}
+
+__device__ __forceinline__ TileMasks load_tile_edges(const int2 *__restrict__ Q, int64_t q, const Blocks &k,
+                                                     int short_max, int64_t t, int lane, float *out_v,
+                                                     int32_t *out_i) {
+    return classify_edges(fetch_query(Q, q, t, lane), q, k, short_max, t, lane, out_v, out_i);
 }
 
 // Scans pairs P[a..b] (each a (min, leftmost argmin) of consecutive elements,
