@@ -55,6 +55,9 @@ class PowerSampler {
         printf("power: %.3f J over %.3f s, average %.2f W\n", energy_j_, time_s_, time_s_ > 0 ? energy_j_ / time_s_ : 0.0);
     }
 
+    // Energy accumulated between start() and stop(), joules (0 without a sensor).
+    double energy_j() const { return energy_j_; }
+
    private:
     static double read_watts(const std::string &sensor) {
         FILE *f = fopen(sensor.c_str(), "r");

@@ -398,7 +398,13 @@ int main(int argc, char **argv) {
             hip_ok(hipDeviceSynchronize(), "query");
         }
         time_ms = now_ms() - t0;
-        if (a.save_power) power.stop();
+        if (a.save_power) {
+            power.stop();
+            // queries answered per joule over the query loop, q * reps / energy
+            // (scripts/plot_rmq_per_joule.py:61-76)
+            if (power.energy_j() > 0)
+                printf("energy efficiency: %.4e RMQ/J\n", (double)a.q * a.reps / power.energy_j());
+        }
         print_rate("", a, time_ms);
         if (idx_out) hip_ok(hipMemcpy(out_i.data(), d_idx, sizeof(int32_t) * (size_t)a.q, hipMemcpyDeviceToHost), "D2H");
         else hip_ok(hipMemcpy(out_v.data(), d_val, sizeof(float) * (size_t)a.q, hipMemcpyDeviceToHost), "D2H");

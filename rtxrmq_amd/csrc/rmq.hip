@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cmath>
@@ -422,18 +423,24 @@ int launch_short(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_
 }
 
 // Whether one workgroup of k_query_edges_lds with `bytes` of LDS fits twice
-// in a CU (full occupancy) and within the per-workgroup limit.
+// in a CU of the current device (full occupancy) and within the
+// per-workgroup limit.  The limits are read once per device.
 bool lds_table_fits(size_t bytes) {
-    static int per_block = -1, per_cu = -1;
-    if (per_block < 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) {
-            per_block = per_cu = 0;
-        }
+    constexpr int kMaxDevices = 64;
+    static std::atomic<int64_t> limits[kMaxDevices];  // (per_block << 32) | per_cu; 0 = not read yet
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return false;
+    int64_t v = limits[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        int per_block = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+            return false;
+        v = ((int64_t)per_block << 32) | (uint32_t)per_cu;
+        limits[dev].store(v, std::memory_order_relaxed);
     }
-    return bytes <= (size_t)per_block && 2 * bytes <= (size_t)per_cu;
+    const size_t per_block = (size_t)(v >> 32), per_cu = (size_t)(uint32_t)v;
+    return bytes <= per_block && 2 * bytes <= per_cu;
 }
 
 // Which kernel answers a handle's long ranges (and, with edge tables, all of
@@ -729,7 +736,7 @@ RMQ_API int rmq_query_kernels(const rmq_handle_t *h, const rmq_query_opts_t *opt
     case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d>", o.group_lanes); break;
     case LongPass::kEdges: w = snprintf(buf, (size_t)len, "k_query_edges<%d>", o.group_lanes); break;
     default:
-        w = snprintf(buf, (size_t)len, "k_query_tiles<%d, false> k_query_tiles<%d, true>", o.group_lanes,
+        w = snprintf(buf, (size_t)len, "k_query_tiles<%d, false>;k_query_tiles<%d, true>", o.group_lanes,
                      o.long_lanes);
         break;
     }

@@ -106,7 +106,7 @@ class RMQIndex:
         buf = ctypes.create_string_buffer(256)
         N.check(N.lib().rmq_query_kernels(self._h, ctypes.byref(opts) if opts is not None else None, buf, 256),
                 "rmq_query_kernels")
-        return [k + ">" if not k.endswith(">") else k for k in buf.value.decode().split("> ") if k]
+        return buf.value.decode().split(";")
 
     def default_opts(self) -> N.QueryOpts:
         o = N.QueryOpts()

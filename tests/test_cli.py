@@ -115,7 +115,9 @@ def test_gpu_edges_and_autotune(alg, tmp_path, gpu):
 def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
     """--save-power samples the amdgpu hwmon sensor every 100 ms during the
     query loop into the reference's six columns (common/nvmlPower.cpp:37-84):
-    timestep, power, accumulated energy, average power, dt, accumulated time."""
+    timestep, power, accumulated energy, average power, dt, accumulated time;
+    and prints RMQ/J as the reference's plots compute it (q * reps / energy,
+    scripts/plot_rmq_per_joule.py:61-76)."""
     pfile = tmp_path / "p.dat"
     r = run(1 << 20, 1 << 16, -1, 2, "--reps", 4, "--seed", 3, f"--save-power={pfile}")
     assert r.returncode == 0, r.stderr[-1500:]
@@ -126,3 +128,5 @@ def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
     vals = np.array(rows, dtype=np.float64)
     assert np.all(vals[:, 1] > 0) and np.all(np.diff(vals[:, 2]) >= 0) and np.all(np.diff(vals[:, 5]) > 0)
     assert "power:" in r.stdout
+    m = re.search(r"energy efficiency: ([0-9.e+]+) RMQ/J", r.stdout)  # q * reps / J
+    assert m and float(m.group(1)) > 0
