@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
 // answer_class.  The short-class kernel also writes (NaN, -1) for invalid
 // queries.  Persistent grid, static stride over tiles.
 template <int G, bool LONG>
-__global__ __launch_bounds__(kWG) void k_query_tiles(Blocks k, const int2 *__restrict__ Q, int64_t q, int short_max,
+__global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_tiles(Blocks k, const int2 *__restrict__ Q, int64_t q, int short_max,
                                                      float *__restrict__ out_v, int32_t *__restrict__ out_i) {
     __shared__ TileStage stage[kWavesPerWG];
     const int lane = threadIdx.x & (kWave - 1);

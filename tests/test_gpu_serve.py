@@ -40,10 +40,13 @@ def request_payloads(arrays, count, seed):
     return reqs
 
 
-@pytest.mark.parametrize("flags", [0, R.BUILD_EDGES])
+@pytest.mark.parametrize("flags", [0, R.BUILD_EDGES, "mixed"])
 def test_alone_and_in_the_mix(arrays, flags, gpu):
+    """Answers are the same alone and coalesced with others; with "mixed",
+    arrays with and without edge tables share launches."""
     with R.Server(slots=3) as sv:
-        ids = [sv.add_array(Ad, bs, flags) for (_, Ad), bs in zip(arrays, (64, 1000, 256, 3))]
+        ids = [sv.add_array(Ad, bs, (R.BUILD_EDGES if k % 2 else 0) if flags == "mixed" else flags)
+               for k, ((_, Ad), bs) in enumerate(zip(arrays, (64, 1000, 256, 3)))]
         reqs = request_payloads(arrays, 48, 5)
         # request 0 alone
         a0, Q0 = reqs[0]
