@@ -92,6 +92,28 @@ class BlockLevel:
         t = yv < xv
         return torch.where(t, yv, xv), torch.where(t, yi, xi)
 
+    def superset(self, lB, rB, lines: bool):
+        """(value, index, table entries read) of the superset test of blocks
+        lB < rB: the minimum of lB..rB from two level-lev entries, or, with
+        lines (a global table, 256-byte aligned), when those two lie in
+        different 128-byte lines and the window fits, the one level-(lev+1)
+        entry at lB (blocks lB .. lB + 2^(lev+1) - 1)."""
+        import torch
+        c = rB - lB + 1
+        v, i = self.range_min(lB, c)
+        reads = torch.full_like(lB, 2)
+        if lines:
+            lev = (torch.frexp(c.double())[1] - 1).long()
+            base = lev * self.nb
+            a2 = rB - torch.bitwise_left_shift(torch.ones_like(lev), lev) + 1
+            split = ((base + lB) * 8) // 128 != ((base + a2) * 8) // 128
+            one = split & (lB + torch.bitwise_left_shift(torch.full_like(lev, 2), lev) <= self.nb)
+            if bool(one.any()):
+                v[one] = self.V[lev[one] + 1, lB[one]]
+                i[one] = self.I[lev[one] + 1, lB[one]]
+            reads = torch.where(one, 1, 2)
+        return v, i, reads
+
 
 HIER_MIN = 4  # kHierMin: whole sub-blocks between the ends before a scan reads their minima
 
@@ -133,7 +155,7 @@ def left_part_min(A, sub, bs: int, l):
 
 
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
-                      sub: BlockLevel | None = None, A=None) -> dict:
+                      sub: BlockLevel | None = None, A=None, lines: bool = False) -> dict:
     """Per pass: bytes in the long and the short class, their query counts,
     the fraction of cross-block edge parts computed, the random gathers of
     per-element edge pairs (with edge tables), the sparse-table entries read
@@ -157,7 +179,10 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     edge tables.  With edge tables the left pair, once read, counts in the best so
     far before the right part is decided; a scanned left part (spread over a
     lane group) does not.  T = 16 when whole blocks (or sub-blocks) lie
-    between.  sub: the 32-element level and A the array, with edge tables."""
+    between.  sub: the 32-element level and A the array, with edge tables.
+    lines: the superset test reads one level-up entry instead of two entries
+    in different 128-byte lines (BlockLevel.superset), as the edge-table
+    kernel with its table in global memory does."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
@@ -189,13 +214,13 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             long_b += float((8 + scan_bytes(l[same], r[same], bs) + out_bytes).sum())
         x = ~same
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
-        sup_i = lvl.range_min(lB, rB - lB + 1)[1]
+        _, sup_i, sr = lvl.superset(lB, rB, edges and lines)
         fast = (sup_i >= l) & (sup_i <= r)
-        long_b += float((8 + 16 + out_bytes) * fast.sum())
+        long_b += float((8 + 8 * sr[fast] + out_bytes).sum())
         parts += 2 * int(fast.sum())
-        reads += 2 * int(l.numel())
+        reads += int(sr.sum())
         x = ~fast
-        l, r, lB, rB = l[x], r[x], lB[x], rB[x]
+        l, r, lB, rB, sr = l[x], r[x], lB[x], rB[x], sr[x]
         mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
         mi = torch.full_like(l, 2 ** 31 - 1)
         mid = rB - lB > 1
@@ -219,7 +244,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         else:
             edge_b = (need_l.long() * scan_bytes(l, (lB + 1) * bs - 1, bs)
                       + need_r.long() * scan_bytes(rB * bs, r, bs))
-        long_b += float((8 + 16 + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
+        long_b += float((8 + 8 * sr + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
             "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads,
             "short_lines": lines}
@@ -428,8 +453,9 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
         e = bool(edges_for and edges_for[lr])
         if e and bs_for[lr] > 32 and bs_for[lr] % 32:
             raise ValueError("the byte model needs block sizes of at most 32 or multiples of 32 with edge tables")
+        lines = bool(e and not (kernels_for and kernels_for[lr][0].startswith("k_query_edges_lds")))
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
-                               sub=levels.get(32) if e else None, A=A)
+                               sub=levels.get(32) if e else None, A=A, lines=lines)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]

@@ -238,7 +238,7 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
 // (answer_edges: no scan, no reduction), then the short ranges with G-lane
 // group scans (answer_class), and writes (NaN, -1) for invalid queries.
 template <int G>
-__global__ __launch_bounds__(kWG) void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
+__global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
                                                      int short_max, float *__restrict__ out_v,
                                                      int32_t *__restrict__ out_i) {
     __shared__ TileStage stage[kWavesPerWG];
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
         const TileMasks m = classify_edges(cur, q, kl, short_max, t, lane, out_v, out_i);
         cur = nxt;
         if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges(e, kl, m.l, m.r);
+            const Pair best = answer_edges<false>(e, kl, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
         if (m.shorts) answer_class<G, false>(kl, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);

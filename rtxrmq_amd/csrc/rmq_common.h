@@ -345,18 +345,38 @@ struct EdgeTables {
 // parts one level down (the range spans at least two sub-blocks of the
 // block).  Parts are combined left to right with strict '<', so the
 // leftmost minimum wins.  One lane per query: no reduction needed.
+// LINES: the table is in global memory (base 256-byte aligned), where each
+// read costs a 128-byte line (see the superset test below).
+template <bool LINES = true>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
     const Pair *__restrict__ st = k.st;
     const int nb = k.nb, bs = k.bs;
     const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
     Pair best, right;
     if (bl != br) {
-        // The minimum of blocks bl..br, when its leftmost argmin lies in
-        // [l, r], is the answer.  Otherwise: whole blocks between, then each
-        // edge from its block's (min, leftmost argmin) in table row 0: exact
-        // when the argmin lies in the edge part, skipped when the block
-        // minimum cannot win, and only otherwise read from the edge table.
-        const Pair sup = table_min(st, nb, bl, br - bl + 1);
+        // The minimum of a window of whole blocks covering bl..br, when its
+        // leftmost argmin lies in [l, r], is the answer (every element of
+        // [l, r] before it is strictly larger).  The window is bl..br, two
+        // table reads; when those two entries lie in different 128-byte
+        // lines, it is the one level-(lev+1) entry at bl instead (blocks
+        // bl..bl+2^(lev+1)-1, up to twice as many), one line instead of two.
+        // Otherwise: whole blocks between, then each edge from its block's
+        // (min, leftmost argmin) in table row 0: exact when the argmin lies
+        // in the edge part, skipped when the block minimum cannot win, and
+        // only otherwise read from the edge table.
+        const int c = br - bl + 1;
+        const int lev = 31 - __clz(c);
+        const Pair *row = st + (int64_t)lev * nb;
+        const int a2 = br - (1 << lev) + 1;
+        Pair sup;
+        if (LINES && ((reinterpret_cast<uintptr_t>(row + bl) ^ reinterpret_cast<uintptr_t>(row + a2)) >> 7) != 0 &&
+            (int64_t)bl + ((int64_t)2 << lev) <= nb) {
+            sup = row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
+        } else {
+            const Pair x = row[bl];
+            const Pair y = row[a2];
+            sup = (y.v < x.v) ? y : x;
+        }
         if (sup.i >= l && sup.i <= r) return sup;
         best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
         const Pair el = st[bl];

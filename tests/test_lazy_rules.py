@@ -48,14 +48,29 @@ def block_mins(A, bs):
     return [brute(A, b, min(b + bs, len(A)) - 1) for b in range(0, len(A), bs)]
 
 
-def lazy(A, bs, bm, l, r, left_counts=True):
+def superset_window(nb, lB, rB, lines):
+    """Blocks [lB, hi] whose minimum the superset test reads: lB..rB from two
+    level-lev sparse-table entries, or with lines (a global table, 256-byte
+    aligned, 8-byte entries), when those two entries lie in different 128-byte
+    lines and the window fits, the one level-(lev+1) entry at lB."""
+    lB, rB = int(lB), int(rB)
+    c = rB - lB + 1
+    lev = c.bit_length() - 1
+    a2 = rB - (1 << lev) + 1
+    if lines and ((lev * nb + lB) * 8) // 128 != ((lev * nb + a2) * 8) // 128 and lB + (2 << lev) <= nb:
+        return lB, lB + (2 << lev) - 1
+    return lB, rB
+
+
+def lazy(A, bs, bm, l, r, left_counts=True, lines=False):
     """(answer, parts computed, superset hit) for lB < rB, following rules
     1-3.  left_counts: a computed left part joins the best so far before the
     right part is decided (the edge-table path reads it as one pair); False
     for the scan path, whose left scan is spread over a lane group."""
     lB, rB = l // bs, r // bs
     sup = (INF, IMAX)
-    for b in range(lB, rB + 1):
+    lo, hi = superset_window(len(bm), lB, rB, lines)
+    for b in range(lo, hi + 1):
         sup = lexmin(bm[b], sup)
     if l <= sup[1] <= r:
         return sup, 0, True
@@ -93,16 +108,19 @@ def arrays(rng):
     return out
 
 
-@pytest.mark.parametrize("left_counts", [True, False])
+@pytest.mark.parametrize("left_counts,lines", [(True, False), (False, False), (True, True)])
 @pytest.mark.parametrize("bs", [1, 2, 3, 8, 16, 33])
-def test_rules_give_the_leftmost_minimum(bs, left_counts):
+def test_rules_give_the_leftmost_minimum(bs, left_counts, lines):
+    """lines: the superset window grows to a level-(lev+1) window when the two
+    level-lev entries lie in different 128-byte lines (any window of whole
+    blocks that covers [l, r] is a valid superset)."""
     for A in arrays(np.random.default_rng(bs)):
         bm = block_mins(A, bs)
         for l in range(len(A)):
             for r in range(l, len(A)):
                 if l // bs == r // bs:
                     continue
-                got = lazy(A, bs, bm, l, r, left_counts)[0]
+                got = lazy(A, bs, bm, l, r, left_counts, lines)[0]
                 exp = brute(A, l, r)
                 assert got[1] == exp[1] and np.float32(got[0]).tobytes() == np.float32(exp[0]).tobytes(), \
                     (A.tolist(), bs, l, r, got, exp)
@@ -126,12 +144,13 @@ def test_superset_rule_inside_one_block_over_sub_blocks():
                         assert sup == brute(A, l, r), (bs, l, r)
 
 
-@pytest.mark.parametrize("edges,short_max", [(False, 32), (True, 32), (True, 1)])
-def test_byte_model_replays_the_rules(edges, short_max):
+@pytest.mark.parametrize("edges,short_max,lines", [(False, 32, False), (True, 32, False), (True, 1, False),
+                                                   (True, 1, True), (True, 32, True)])
+def test_byte_model_replays_the_rules(edges, short_max, lines):
     """With edge tables a range inside one 32-element sub-block (one block
     when bs <= 32) is scanned whatever short_max is; every other range longer
     than short_max takes the O(1) path."""
-    rng = np.random.default_rng(11 + edges + short_max)
+    rng = np.random.default_rng(11 + edges + short_max + 5 * lines)
     for trial in range(16):
         n = int(rng.integers(100, 2500))
         bs = int(rng.choice([8, 24, 64, 128, 256] if edges else [4, 16, 37, 64, 256]))
@@ -160,14 +179,14 @@ def test_byte_model_replays_the_rules(edges, short_max):
                     s = brute(A, lo, hi)
                     gathers += 0 if l <= s[1] <= r else 2
                 continue
-            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges)
+            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines)
             parts += 2
             need += k
             gathers += k if edges else 0
         lvl = bench.BlockLevel(torch.from_numpy(A), bs)
         sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
         ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, short_max, edges=edges, sub=sub,
-                                     A=torch.from_numpy(A))
+                                     A=torch.from_numpy(A), lines=lines)
         assert ab["n_short"] == n_short
         assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
         assert ab["gathers"] == gathers
