@@ -85,7 +85,7 @@ void usage() {
             "  --short-max <len>     ranges up to this length use the group scan (alg 5/105)\n"
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
-            "  --edges               build edge tables: O(1) answers for ranges over 32 elements\n"
+            "  --edges               build edge tables: O(1) answers for ranges not inside a 32-element sub-block\n"
             "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^22 queries\n"
             "                        (overrides --bs/--nb/--edges and the lane options; optional pick cache)\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
