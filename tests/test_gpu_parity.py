@@ -6,6 +6,8 @@ Small cases go against brute force and the golden fixtures; the BASELINE
 sizes (n=1e8, q=2^26; n=2^27 stress arrays) go against the O(1) fast oracle
 plus size-independent properties (l <= idx <= r, A[idx] == value).
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -218,6 +220,34 @@ def test_headline_full_size(lr, flags, gpu):
     fv, fi = f.query(Q)
     f.close()
     assert_exact(v, i, fv, fi, f"headline lr={lr}")
+
+
+@pytest.mark.parametrize("lr", [-1, -2, -3])
+def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
+    """The configurations the headline runs: n=1e8, each length distribution
+    at the block size, edge tables and thresholds of the committed MI355X
+    tuner cache (the LDS-table kernel for uniform lengths, the global-table
+    kernel with its line-aware superset test for the lognormal ones), every
+    answer checked against the O(1) oracle."""
+    import shutil
+    n, q = 100_000_000, 1 << 26
+    A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
+    Q = R.gen_queries(q, n, lr, 1321 + lr)
+    Q_dev = _dev(Q, gpu)
+    cache = tmp_path / "picks.txt"  # a copy: a missing key is tuned here, not written to the committed file
+    shutil.copy(os.path.join(os.path.dirname(R.__file__), "tune_cache_mi355x.txt"), cache)
+    pick = R.tune(A_dev, Q_dev[: 1 << 22], cache_path=str(cache))
+    with R.RMQIndex(A_dev, pick["bs"], flags=pick["flags"]) as ix:
+        kern = ix.kernels(pick["opts"])
+        v, i = ix.query(Q_dev, opts=pick["opts"])
+        torch.cuda.synchronize()
+        v, i = v.cpu().numpy(), i.cpu().numpy()
+    del Q_dev
+    A = A_dev.cpu().numpy()
+    f = O.FastOracle(A)
+    fv, fi = f.query(Q)
+    f.close()
+    assert_exact(v, i, fv, fi, f"lr={lr} pick={pick['key']} bs={pick['bs']} kernels={kern}")
 
 
 @pytest.mark.parametrize("flags", FLAGS)
