@@ -866,6 +866,8 @@ def main() -> int:
 
     traffic = load_traffic(tuned["dominant"], workload)
     dom_ms = tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"]
+    fixed_traffic = load_traffic(fixed["dominant"], fixed_workload)
+    fixed_ms = fixed["per_kernel"][fixed["dominant"]]["avg_launch_ms"]
     line = {
         "metric": "RMQ/s at n=1e8,q=2^26 per length dist, %HBM roofline; batched/served vs plain",
         "value": tuned["value"],
@@ -909,8 +911,9 @@ def main() -> int:
             "per_kernel": fixed["per_kernel"],
             "roofline": {"kernel": fixed["dominant"], "achieved": fixed["achieved"],
                          "frac": fixed["achieved"] / HBM_PEAK_GBS,
-                         "traffic": load_traffic(fixed["dominant"], fixed_workload),
-                         "avg_launch_ms": fixed["per_kernel"][fixed["dominant"]]["avg_launch_ms"],
+                         "traffic": fixed_traffic,
+                         "traffic_GBps": fixed_traffic / (fixed_ms / 1e3) / 1e9 if fixed_traffic else None,
+                         "avg_launch_ms": fixed_ms,
                          "workload": fixed_workload}},
     }
     print(json.dumps(line), flush=True)
