@@ -69,7 +69,7 @@ def cases():
     # constant array: leftmost = l, HRMQ = r
     A = np.full(50, 7.0, np.float32)
     out["const_n50_allpairs"] = (A, all_pairs(50), True)
-    # closed forms from the survey: [5,5] and [3,5,3]
+    # closed forms: [5,5] -> leftmost 0 / rightmost 1, [3,5,3] -> leftmost 0 / rightmost 2
     out["closed_5_5"] = (np.array([5, 5], np.float32), all_pairs(2), True)
     out["closed_3_5_3"] = (np.array([3, 5, 3], np.float32), all_pairs(3), True)
     # signed zeros and infinities (outside HRMQ's int32 contract)

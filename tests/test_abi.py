@@ -107,9 +107,10 @@ def test_product_path_has_no_cpu_fallback():
 
 @pytest.mark.parametrize("lang", ["c", "c++"])
 def test_header_compiles_and_links_for_c_and_cpp(lang, tmp_path):
-    """include/rmq.h is the boundary a C or C++ host includes (the drop-in of
-    INTEGRATION.md): it must compile on its own as C11 and C++17, and a program
-    linked against librtxrmq.so must run (calls that need no device)."""
+    """include/rmq.h is the boundary a C or C++ host includes in place of the
+    reference's rtx_rmq<T> (src/rtx.h:24-177): it must compile on its own as
+    C11 and C++17, and a program linked against librtxrmq.so must run (calls
+    that need no device)."""
     import shutil
     import subprocess
     cc = shutil.which("gcc" if lang == "c" else "g++")
