@@ -329,6 +329,40 @@ def test_concurrent_streams_same_handle(gpu):
             assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei)
 
 
+@pytest.mark.parametrize("bs,flags", [(1 << 17, R.BUILD_EDGES), (64, R.BUILD_EDGES), (1 << 12, 0)])
+def test_query_replays_from_a_hip_graph(bs, flags, gpu):
+    """rmq_query allocates nothing and records no memset, so a caller can
+    capture it in a HIP graph and replay it.  Each replay, with new queries
+    copied into the captured input and the outputs cleared by the program's
+    own eager work in between, matches the brute force -- for the three query
+    paths: block table in LDS, in global memory, and no edge tables."""
+    n, qn = 1 << 20, 1 << 15
+    A = O.gen_array(n, 21, "ties16")
+    A_dev = _dev(A, gpu)
+    Qbuf = torch.empty((qn, 2), dtype=torch.int32, device=gpu)
+    v = torch.empty(qn, dtype=torch.float32, device=gpu)
+    i = torch.empty(qn, dtype=torch.int32, device=gpu)
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
+        Qbuf.copy_(_dev(O.gen_queries(qn, n, -1, 1), gpu))
+        ix.query(Qbuf, out_val=v, out_idx=i)  # loads the kernels before capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            ix.query(Qbuf, out_val=v, out_idx=i)
+        for k, lr in enumerate((-1, -2, -3, 5, -1)):
+            Q = O.gen_queries(qn, n, lr, 100 + k)
+            Qbuf.copy_(_dev(Q, gpu))
+            v.fill_(0.0)
+            i.fill_(-7)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            ev, ei = O.rmq_brute(A, Q)
+            assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, f"replay {k} bs={bs} flags={flags}")
+        kern = ix.kernels()
+    assert kern[0].startswith({(1 << 17): "k_query_edges_lds", 64: "k_query_edges<", 1 << 12: "k_query_tiles"}[bs])
+
+
 def test_rtx_rmq_mirror(gpu):
     n, q, bs = 1 << 16, 1 << 14, 1 << 10
     A_dev = R.gen_array(n, 3, device=gpu)
