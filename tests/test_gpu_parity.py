@@ -311,12 +311,15 @@ def test_device_generator_matches_host(gpu):
         np.testing.assert_array_equal(d.view(np.int32), R.gen_array_host(1 << 20, 1234, kind).view(np.int32))
 
 
-def test_concurrent_streams_same_handle(gpu):
+@pytest.mark.parametrize("bs,flags", [(1 << 12, 0), (1 << 15, R.BUILD_EDGES), (64, R.BUILD_EDGES)])
+def test_concurrent_streams_same_handle(bs, flags, gpu):
+    """One read-only handle queried from three streams at once (tile kernels,
+    LDS-table and global-table edge kernels)."""
     A = O.gen_array(1 << 20, 8)
     Qs = [O.gen_queries(1 << 16, A.size, lr, 8) for lr in (-1, -2, -3)]
     A_dev = _dev(A, gpu)
     streams = [torch.cuda.Stream() for _ in Qs]
-    with R.RMQIndex(A_dev, 1 << 12) as ix:
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
         torch.cuda.synchronize()
         outs = []
         for s, Q in zip(streams, Qs):
