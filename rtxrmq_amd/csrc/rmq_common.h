@@ -141,11 +141,13 @@ __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, i
 //
 // Binning is done per tile of 64 consecutive queries, in registers: lane j of
 // a wave loads query t*64+j, and two ballots split the tile into short ranges
-// (at most short_max elements) and long ones.  The short kernel answers the
-// short ranges of every tile, several per wave; the long kernel answers the
-// long ranges of every tile, one whole wave each.  Both read the query array
-// once, so binning costs 8 bytes per query per kernel and no atomics, no
-// queue and no workspace: each query writes only its own output slot.
+// (scanned by lane groups) and long ones.  Without edge tables two kernels
+// run: the short one answers the short ranges of every tile, several per
+// wave; the long one the long ranges, a lane group each.  With edge tables
+// one kernel answers both classes of a tile (long ones one lane per query).
+// Each kernel reads the query array once, so binning costs 8 bytes per query
+// per kernel and no atomics, no queue and no workspace: each query writes
+// only its own output slot.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
