@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
         const TileMasks m = classify_edges(cur, q, kl, short_max, t, lane, out_v, out_i);
         cur = nxt;
         if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges<false>(e, kl, m.l, m.r);
+            const Pair best = answer_edges<false, true>(e, kl, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
         if (m.shorts) answer_class<G, false>(kl, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
