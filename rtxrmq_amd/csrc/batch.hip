@@ -64,16 +64,17 @@ __global__ __launch_bounds__(kWG) void k_plan_query(const ArrayDesc *__restrict_
         const bool edges = d.edges.pre_b != nullptr;  // uniform across the wave
         const int64_t tiles = ((int64_t)w.count + kWave - 1) / kWave;
         for (int64_t t = 0; t < tiles; ++t) {
-            const TileMasks m = edges ? load_tile_edges(w.Q, w.count, d.k, short_max, t, lane, w.out_v, w.out_i)
-                                      : load_tile(w.Q, w.count, d.k.n, short_max, t, lane, true, w.out_v, w.out_i);
-            answer_class<GS, false>(d.k, m.shorts, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
+            // ordinary loads and stores: a server's work items may point at pinned host memory
+            const TileMasks m = edges ? load_tile_edges<false>(w.Q, w.count, d.k, short_max, t, lane, w.out_v, w.out_i)
+                                      : load_tile<false>(w.Q, w.count, d.k.n, short_max, t, lane, true, w.out_v, w.out_i);
+            answer_class<GS, false, false>(d.k, m.shorts, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
             if (edges) {
                 if ((m.longs >> lane) & 1) {
                     const Pair b = answer_edges(d.edges, d.k, m.l, m.r);
-                    write_result(w.out_v, w.out_i, t * kWave + lane, b.v, b.i);
+                    write_result<false>(w.out_v, w.out_i, t * kWave + lane, b.v, b.i);
                 }
             } else {
-                answer_class<GL, true>(d.k, m.longs, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
+                answer_class<GL, true, false>(d.k, m.longs, m, lane, t * kWave, w.out_v, w.out_i, stage[wv]);
             }
         }
     }

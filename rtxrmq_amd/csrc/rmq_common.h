@@ -157,18 +157,33 @@ __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, i
 // ---------------------------------------------------------------------------
 
 // Results are written once and not read again by the kernels, so they are
-// stored non-temporally: they do not displace table lines from the caches
-// (+2% on the three n=1e8 batches, most on the L2-resident lognormal 2^14 one).
+// stored non-temporally (NT): they do not displace table lines from the
+// caches (+2% on the three n=1e8 batches, most on the L2-resident lognormal
+// 2^14 one).  The plan kernel uses ordinary stores (NT = false): a server
+// hands it pinned host buffers, which it reads and writes over PCIe, where
+// the hint has nothing to gain (its batched rate is the same either way).
+template <bool NT = true>
 __device__ __forceinline__ void write_result(float *out_v, int32_t *out_i, int64_t qi, float v, int i) {
-    if (out_v) __builtin_nontemporal_store(v, out_v + qi);
-    if (out_i) __builtin_nontemporal_store(i, out_i + qi);
+    if constexpr (NT) {
+        if (out_v) __builtin_nontemporal_store(v, out_v + qi);
+        if (out_i) __builtin_nontemporal_store(i, out_i + qi);
+    } else {
+        if (out_v) out_v[qi] = v;
+        if (out_i) out_i[qi] = i;
+    }
 }
 
 // The query stream is read once per pass: a non-temporal load keeps its lines
-// from displacing table lines in the caches (+2-3% on the n=1e8 batches).
-__device__ __forceinline__ int2 load_query_nt(const int2 *__restrict__ Q, int64_t qi) {
-    const long long raw = __builtin_nontemporal_load(reinterpret_cast<const long long *>(Q) + qi);
-    return make_int2((int)(uint32_t)raw, (int)(uint32_t)((unsigned long long)raw >> 32));
+// from displacing table lines in the caches (+2-3% on the n=1e8 batches);
+// NT = false as for write_result.
+template <bool NT = true>
+__device__ __forceinline__ int2 load_query(const int2 *__restrict__ Q, int64_t qi) {
+    if constexpr (NT) {
+        const long long raw = __builtin_nontemporal_load(reinterpret_cast<const long long *>(Q) + qi);
+        return make_int2((int)(uint32_t)raw, (int)(uint32_t)((unsigned long long)raw >> 32));
+    } else {
+        return Q[qi];
+    }
 }
 
 struct TileMasks {
@@ -177,18 +192,19 @@ struct TileMasks {
     uint64_t longs;    // lanes holding a valid longer range
 };
 
+template <bool NT = true>
 __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64_t q, int n, int short_max, int64_t t,
                                                int lane, bool flag_invalid, float *out_v, int32_t *out_i) {
     const int64_t qi = t * kWave + lane;
     const bool in = qi < q;
     int l = 0, r = -1;
     if (in) {
-        const int2 p = load_query_nt(Q, qi);
+        const int2 p = load_query<NT>(Q, qi);
         l = p.x;
         r = p.y;
     }
     const bool ok = in && l >= 0 && l <= r && r < n;
-    if (flag_invalid && in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    if (flag_invalid && in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     const bool is_short = ok && (r - l) < short_max;
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
 }
@@ -226,19 +242,21 @@ struct Blocks {
 // those, and ranges of at most short_max elements, go to the group scan.
 // Otherwise as load_tile, with invalid queries flagged.
 // Query t*64+lane, or the empty range (0, -1) past the end.
+template <bool NT = true>
 __device__ __forceinline__ int2 fetch_query(const int2 *__restrict__ Q, int64_t q, int64_t t, int lane) {
     const int64_t qi = t * kWave + lane;
-    return qi < q ? load_query_nt(Q, qi) : make_int2(0, -1);
+    return qi < q ? load_query<NT>(Q, qi) : make_int2(0, -1);
 }
 
 // load_tile_edges on a query already loaded (p = fetch_query(Q, q, t, lane)).
+template <bool NT = true>
 __device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blocks &k, int short_max, int64_t t,
                                                     int lane, float *out_v, int32_t *out_i) {
     const int64_t qi = t * kWave + lane;
     const bool in = qi < q;
     const int l = p.x, r = p.y;
     const bool ok = in && l >= 0 && l <= r && r < k.n;
-    if (in && !ok) write_result(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    if (in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     bool is_short = ok && (r - l) < short_max;
     if (ok && !is_short && r - l < kSub) {
         const int bl = block_of(l, k.magic);
@@ -248,10 +266,11 @@ __device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blo
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
 }
 
+template <bool NT = true>
 __device__ __forceinline__ TileMasks load_tile_edges(const int2 *__restrict__ Q, int64_t q, const Blocks &k,
                                                      int short_max, int64_t t, int lane, float *out_v,
                                                      int32_t *out_i) {
-    return classify_edges(fetch_query(Q, q, t, lane), q, k, short_max, t, lane, out_v, out_i);
+    return classify_edges<NT>(fetch_query<NT>(Q, q, t, lane), q, k, short_max, t, lane, out_v, out_i);
 }
 
 // Scans pairs P[a..b] (each a (min, leftmost argmin) of consecutive elements,
@@ -446,7 +465,7 @@ struct TileStage {
 // wave per query, queries taken from the mask in lane order.  G < 64: the
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
-template <int G, bool LONG>
+template <int G, bool LONG, bool NT = true>
 __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage) {
@@ -461,7 +480,7 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             if (lB != rB) {
                 const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
                 if (sup.i >= m.l && sup.i <= m.r) {
-                    write_result(out_v, out_i, qbase + lane, sup.v, sup.i);
+                    write_result<NT>(out_v, out_i, qbase + lane, sup.v, sup.i);
                     done = true;
                 }
             }
@@ -480,7 +499,7 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             take_if_better(mv, mi, bv, bi);
             if (lane == 0) {
                 if (bv == INFINITY) bi = l;
-                write_result(out_v, out_i, qbase + src, bv, bi);
+                write_result<NT>(out_v, out_i, qbase + src, bv, bi);
             }
         }
     } else {
@@ -510,7 +529,7 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             take_if_better(mv, mi, bv, bi);
             if (has && gl == 0) {
                 if (bv == INFINITY) bi = ql;
-                write_result(out_v, out_i, qbase + stage.lane[s], bv, bi);
+                write_result<NT>(out_v, out_i, qbase + stage.lane[s], bv, bi);
             }
         }
         __builtin_amdgcn_wave_barrier();
