@@ -155,7 +155,7 @@ def left_part_min(A, sub, bs: int, l):
 
 
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
-                      sub: BlockLevel | None = None, A=None, lines: bool = False) -> dict:
+                      sub: BlockLevel | None = None, A=None, lines: bool = False, two_stage: bool = False) -> dict:
     """Per pass: bytes in the long and the short class, their query counts,
     the fraction of cross-block edge parts computed, the random gathers of
     per-element edge pairs (with edge tables), the sparse-table entries read
@@ -176,7 +176,9 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     block's leftmost argmin lies inside it (then that entry is exact) or its
     block minimum cannot win (left: > the best so far; right: >= it); a needed
     part costs its scan's bytes (scan_bytes), or 8 B (one edge pair) with
-    edge tables.  With edge tables the left pair, once read, counts in the best so
+    edge tables.  two_stage (edge tables, large tables): a failed superset
+    test is first retried with the failing side cut off (16 B); when that
+    answers, one edge pair (8 B) completes it.  With edge tables the left pair, once read, counts in the best so
     far before the right part is decided; a scanned left part (spread over a
     lane group) does not.  T = 16 when whole blocks (or sub-blocks) lie
     between.  sub: the 32-element level and A the array, with edge tables.
@@ -220,7 +222,25 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         parts += 2 * int(fast.sum())
         reads += int(sr.sum())
         x = ~fast
-        l, r, lB, rB, sr = l[x], r[x], lB[x], rB[x], sr[x]
+        l, r, lB, rB, sr, sup_i = l[x], r[x], lB[x], rB[x], sr[x], sup_i[x]
+        s2b = torch.zeros_like(l)  # bytes of a second superset test that did not answer
+        if edges and two_stage:
+            # second test with the failed side cut off: argmin before l (blocks
+            # lB+1..rB, then the left edge pair) or in block rB after r (blocks
+            # lB..rB-1, then the right edge pair)
+            left = sup_i < l
+            right = ~left & (sup_i < (rB + 1) * bs)
+            tried = left | right
+            _, s2i = lvl.range_min(torch.where(left, lB + 1, lB), rB - lB)
+            ok2 = (left & (s2i <= r)) | (right & (s2i >= l))
+            reads += 2 * int(tried.sum())
+            gathers += int(ok2.sum())
+            parts += 2 * int(ok2.sum())
+            needed += int(ok2.sum())
+            long_b += float((8 + 8 * sr[ok2] + 16 + 8 + out_bytes).sum())
+            x = ~ok2
+            l, r, lB, rB, sr = l[x], r[x], lB[x], rB[x], sr[x]
+            s2b = 16 * tried[x].long()
         mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
         mi = torch.full_like(l, 2 ** 31 - 1)
         mid = rB - lB > 1
@@ -244,7 +264,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         else:
             edge_b = (need_l.long() * scan_bytes(l, (lB + 1) * bs - 1, bs)
                       + need_r.long() * scan_bytes(rB * bs, r, bs))
-        long_b += float((8 + 8 * sr + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
+        long_b += float((8 + 8 * sr + s2b + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
             "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads,
             "short_lines": lines}
@@ -453,9 +473,11 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
         e = bool(edges_for and edges_for[lr])
         if e and bs_for[lr] > 32 and bs_for[lr] % 32:
             raise ValueError("the byte model needs block sizes of at most 32 or multiples of 32 with edge tables")
-        lines = bool(e and not (kernels_for and kernels_for[lr][0].startswith("k_query_edges_lds")))
+        kname = kernels_for[lr][0] if kernels_for else ""
+        lines = bool(e and not kname.startswith("k_query_edges_lds"))
+        two = bool(e and kname.startswith("k_query_edges<") and kname.endswith(", true>"))
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
-                               sub=levels.get(32) if e else None, A=A, lines=lines)
+                               sub=levels.get(32) if e else None, A=A, lines=lines, two_stage=two)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]

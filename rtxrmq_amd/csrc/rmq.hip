@@ -237,7 +237,8 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
 // are not short (load_tile_edges) one lane per query by O(1) lookups
 // (answer_edges: no scan, no reduction), then the short ranges with G-lane
 // group scans (answer_class), and writes (NaN, -1) for invalid queries.
-template <int G>
+// TWO: failed superset tests are retried (answer_edges), for large tables.
+template <int G, bool TWO>
 __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
                                                      int short_max, float *__restrict__ out_v,
                                                      int32_t *__restrict__ out_i) {
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, E
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
         const TileMasks m = load_tile_edges(Q, q, k, short_max, t, lane, out_v, out_i);
         if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges(e, k, m.l, m.r);
+            const Pair best = answer_edges<true, false, TWO>(e, k, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
         if (m.shorts) answer_class<G, false>(k, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
@@ -449,6 +450,10 @@ enum class LongPass { kTiles, kEdges, kEdgesLds };
 
 size_t lds_table_bytes(const rmq_handle *h) { return (size_t)h->levels * (size_t)h->nb * sizeof(Pair); }
 
+// Whether the global-table kernel retries failed superset tests: only for
+// tables far beyond the caches, where its saved DRAM lines pay.
+bool two_stage(const rmq_handle *h) { return h->st_bytes > kTwoStageTableBytes; }
+
 LongPass long_pass(const rmq_handle *h) {
     if (!h->edges.pre_b) return LongPass::kTiles;
     return lds_table_fits(lds_table_bytes(h) + sizeof(TileStage) * kLdsWaves) ? LongPass::kEdgesLds
@@ -477,11 +482,20 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
             }
             return hip_rc(hipGetLastError());
         }
-        switch (o.group_lanes) {
-        case 4: k_query_edges<4><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
-        case 8: k_query_edges<8><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
-        case 16: k_query_edges<16><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
-        default: k_query_edges<64><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+        if (two_stage(h)) {
+            switch (o.group_lanes) {
+            case 4: k_query_edges<4, true><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            case 8: k_query_edges<8, true><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            case 16: k_query_edges<16, true><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            default: k_query_edges<64, true><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            }
+        } else {
+            switch (o.group_lanes) {
+            case 4: k_query_edges<4, false><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            case 8: k_query_edges<8, false><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            case 16: k_query_edges<16, false><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            default: k_query_edges<64, false><<<grid, kWG, 0, s>>>(k, h->edges, Q, q, sm, val, idx); break;
+            }
         }
         return hip_rc(hipGetLastError());
     }
@@ -734,7 +748,9 @@ RMQ_API int rmq_query_kernels(const rmq_handle_t *h, const rmq_query_opts_t *opt
     int w = 0;
     switch (long_pass(h)) {
     case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d>", o.group_lanes); break;
-    case LongPass::kEdges: w = snprintf(buf, (size_t)len, "k_query_edges<%d>", o.group_lanes); break;
+    case LongPass::kEdges:
+        w = snprintf(buf, (size_t)len, "k_query_edges<%d, %s>", o.group_lanes, two_stage(h) ? "true" : "false");
+        break;
     default:
         w = snprintf(buf, (size_t)len, "k_query_tiles<%d, false>;k_query_tiles<%d, true>", o.group_lanes,
                      o.long_lanes);

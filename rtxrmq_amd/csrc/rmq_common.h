@@ -19,6 +19,7 @@ constexpr int kWave = 64;
 constexpr int kWG = 256;                 // threads per workgroup (4 waves)
 constexpr int kWavesPerWG = kWG / kWave;
 constexpr int kLdsWaves = 16;            // waves per workgroup of the LDS-table query kernel
+constexpr int64_t kTwoStageTableBytes = int64_t(64) << 20;  // block tables beyond this retry failed tests
 // SGPR budget for query kernels that would otherwise allocate more than 80
 // SGPRs: on gfx950 an allocation of 96 or more cuts resident waves by 40-50%
 // (measured with SQ_WAVE_CYCLES / SQ_BUSY_CYCLES) although the compiler still
@@ -387,7 +388,12 @@ struct EdgeTables {
 // per-element edge pairs are loaded non-temporally (measured: uniform lengths
 // with the table in LDS 0.386 -> 0.342 ms; with the table in global memory it
 // costs the lognormal 2^14 batch 10%, so that kernel keeps cached loads).
-template <bool LINES = true, bool NT_PAIRS = false>
+// TWO_STAGE: a failed superset test is retried with the failing side cut off
+// (below).  It saves DRAM lines where the table misses the caches (lognormal
+// 2^6 lengths at bs=8, a 2.4 GB table: 2.14 -> 2.02 ms) and costs where it
+// hits them (lognormal 2^14 at bs=2048, 6 MB: +3.5%), so the host turns it on
+// for large tables only (kTwoStageTableBytes).
+template <bool LINES = true, bool NT_PAIRS = false, bool TWO_STAGE = false>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
     const Pair *__restrict__ st = k.st;
     const int nb = k.nb, bs = k.bs;
@@ -418,6 +424,28 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &
             sup = (y.v < x.v) ? y : x;
         }
         if (sup.i >= l && sup.i <= r) return sup;
+        // Second test, with the side that failed cut off: the argmin lies in
+        // block bl before l (blocks bl+1..br are tried, plus the left edge
+        // pair) or in block br after r (blocks bl..br-1, plus the right edge
+        // pair).  The shrunk window's argmin inside [l, r] makes it the
+        // answer for the whole blocks, and the one edge pair completes it.
+        if constexpr (TWO_STAGE) {
+            if (sup.i < l) {
+                const Pair s2 = table_min(st, nb, bl + 1, br - bl);
+                if (s2.i <= r) {
+                    Pair x;
+                    if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
+                    return (x.v <= s2.v) ? x : s2;  // the left part's indices are smaller: it wins ties
+                }
+            } else if ((int64_t)sup.i < (int64_t)(br + 1) * bs) {
+                const Pair s2 = table_min(st, nb, bl, br - bl);
+                if (s2.i >= l) {
+                    Pair x;
+                    if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
+                    return (x.v < s2.v) ? x : s2;  // the right part's indices are larger: strictly smaller wins
+                }
+            }
+        }
         best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
         const Pair el = st[bl];
         if (el.i >= l) {

@@ -62,18 +62,37 @@ def superset_window(nb, lB, rB, lines):
     return lB, rB
 
 
-def lazy(A, bs, bm, l, r, left_counts=True, lines=False):
+def blocks_min(bm, lo, hi):
+    best = (INF, IMAX)
+    for b in range(lo, hi + 1):
+        best = lexmin(bm[b], best)
+    return best
+
+
+def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False):
     """(answer, parts computed, superset hit) for lB < rB, following rules
     1-3.  left_counts: a computed left part joins the best so far before the
     right part is decided (the edge-table path reads it as one pair); False
-    for the scan path, whose left scan is spread over a lane group."""
+    for the scan path, whose left scan is spread over a lane group.
+    two_stage: a failed superset test is retried with the failing side cut
+    off -- argmin before l: blocks lB+1..rB, and if that argmin is <= r the
+    answer is it and the left part; argmin in block rB after r: blocks
+    lB..rB-1 and the right part (one part computed) -- as the edge-table
+    kernels do."""
     lB, rB = l // bs, r // bs
-    sup = (INF, IMAX)
     lo, hi = superset_window(len(bm), lB, rB, lines)
-    for b in range(lo, hi + 1):
-        sup = lexmin(bm[b], sup)
+    sup = blocks_min(bm, lo, hi)
     if l <= sup[1] <= r:
         return sup, 0, True
+    if two_stage:
+        if sup[1] < l:
+            s2 = blocks_min(bm, lB + 1, rB)
+            if s2[1] <= r:
+                return lexmin(brute(A, l, lB * bs + bs - 1), s2), 1, False
+        elif sup[1] < (rB + 1) * bs:
+            s2 = blocks_min(bm, lB, rB - 1)
+            if s2[1] >= l:
+                return lexmin(s2, brute(A, rB * bs, r)), 1, False
     best = (INF, IMAX)
     for b in range(lB + 1, rB):
         best = lexmin(bm[b], best)
@@ -108,19 +127,22 @@ def arrays(rng):
     return out
 
 
-@pytest.mark.parametrize("left_counts,lines", [(True, False), (False, False), (True, True)])
+@pytest.mark.parametrize("left_counts,lines,two_stage", [(True, False, False), (False, False, False),
+                                                         (True, True, False), (True, False, True),
+                                                         (True, True, True)])
 @pytest.mark.parametrize("bs", [1, 2, 3, 8, 16, 33])
-def test_rules_give_the_leftmost_minimum(bs, left_counts, lines):
+def test_rules_give_the_leftmost_minimum(bs, left_counts, lines, two_stage):
     """lines: the superset window grows to a level-(lev+1) window when the two
     level-lev entries lie in different 128-byte lines (any window of whole
-    blocks that covers [l, r] is a valid superset)."""
+    blocks that covers [l, r] is a valid superset); two_stage: a failed test
+    is retried with the failing side cut off."""
     for A in arrays(np.random.default_rng(bs)):
         bm = block_mins(A, bs)
         for l in range(len(A)):
             for r in range(l, len(A)):
                 if l // bs == r // bs:
                     continue
-                got = lazy(A, bs, bm, l, r, left_counts, lines)[0]
+                got = lazy(A, bs, bm, l, r, left_counts, lines, two_stage)[0]
                 exp = brute(A, l, r)
                 assert got[1] == exp[1] and np.float32(got[0]).tobytes() == np.float32(exp[0]).tobytes(), \
                     (A.tolist(), bs, l, r, got, exp)
@@ -144,13 +166,14 @@ def test_superset_rule_inside_one_block_over_sub_blocks():
                         assert sup == brute(A, l, r), (bs, l, r)
 
 
-@pytest.mark.parametrize("edges,short_max,lines", [(False, 32, False), (True, 32, False), (True, 1, False),
-                                                   (True, 1, True), (True, 32, True)])
-def test_byte_model_replays_the_rules(edges, short_max, lines):
+@pytest.mark.parametrize("edges,short_max,lines,two_stage", [(False, 32, False, False), (True, 32, False, False),
+                                                             (True, 1, False, False), (True, 1, True, False),
+                                                             (True, 32, True, True), (True, 1, False, True)])
+def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
     """With edge tables a range inside one 32-element sub-block (one block
     when bs <= 32) is scanned whatever short_max is; every other range longer
     than short_max takes the O(1) path."""
-    rng = np.random.default_rng(11 + edges + short_max + 5 * lines)
+    rng = np.random.default_rng(11 + edges + short_max + 5 * lines + 7 * two_stage)
     for trial in range(16):
         n = int(rng.integers(100, 2500))
         bs = int(rng.choice([8, 24, 64, 128, 256] if edges else [4, 16, 37, 64, 256]))
@@ -179,14 +202,14 @@ def test_byte_model_replays_the_rules(edges, short_max, lines):
                     s = brute(A, lo, hi)
                     gathers += 0 if l <= s[1] <= r else 2
                 continue
-            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines)
+            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines, two_stage=two_stage)
             parts += 2
             need += k
             gathers += k if edges else 0
         lvl = bench.BlockLevel(torch.from_numpy(A), bs)
         sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
         ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, short_max, edges=edges, sub=sub,
-                                     A=torch.from_numpy(A), lines=lines)
+                                     A=torch.from_numpy(A), lines=lines, two_stage=two_stage)
         assert ab["n_short"] == n_short
         assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
         assert ab["gathers"] == gathers
