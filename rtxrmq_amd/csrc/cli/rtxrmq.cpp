@@ -86,7 +86,7 @@ void usage() {
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
             "  --edges               build edge tables: O(1) answers for ranges not inside a 32-element sub-block\n"
-            "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^22 queries\n"
+            "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^24 queries\n"
             "                        (overrides --bs/--nb/--edges and the lane options; optional pick cache)\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
             "  --qgen <gen>          ours (default: counter-based, thread-count independent) |\n"
@@ -327,7 +327,8 @@ int main(int argc, char **argv) {
     rmq_query_opts_t opts{a.short_max, a.group_lanes, 0, a.long_lanes};
     if (engine && a.tune) {
         // timed passes over a prefix of the queries; answers do not depend on the pick
-        const int64_t sample = std::min<int64_t>(a.q, (int64_t)1 << 22);
+        // 2^24 queries: at 2^22 a timed pass takes ~20 us and the picks are noisy
+        const int64_t sample = std::min<int64_t>(a.q, (int64_t)1 << 24);
         float *d_sv = nullptr;
         int32_t *d_si = nullptr;
         hip_ok(hipMalloc(&d_sv, sizeof(float) * (size_t)sample), "hipMalloc(tune)");
