@@ -377,108 +377,125 @@ struct EdgeTables {
 };
 
 
+// The superset window's minimum for a range across blocks bl < br: blocks
+// bl..br from two level-lev entries, or, with LINES, when those two entries
+// lie in different 128-byte lines, the one level-(lev+1) entry at bl (blocks
+// bl..bl+2^(lev+1)-1, up to twice as many): one line instead of two.
+template <bool LINES>
+__device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb, int bl, int br) {
+    const int c = br - bl + 1;
+    const int lev = 31 - __clz(c);
+    const Pair *row = st + (int64_t)lev * nb;
+    const int a2 = br - (1 << lev) + 1;
+    if (LINES && ((reinterpret_cast<uintptr_t>(row + bl) ^ reinterpret_cast<uintptr_t>(row + a2)) >> 7) != 0 &&
+        (int64_t)bl + ((int64_t)2 << lev) <= nb) {
+        return row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
+    }
+    const Pair x = row[bl];
+    const Pair y = row[a2];
+    return (y.v < x.v) ? y : x;
+}
+
+// A range across blocks bl < br whose superset window minimum sup has its
+// argmin outside [l, r].
+template <bool NT_PAIRS, bool TWO_STAGE>
+__device__ __forceinline__ Pair answer_failed(const EdgeTables &e, const Blocks &k, int l, int r, int bl, int br,
+                                              Pair sup) {
+    const Pair *__restrict__ st = k.st;
+    const int nb = k.nb, bs = k.bs;
+    // Second test, with the side that failed cut off: the argmin lies in
+    // block bl before l (blocks bl+1..br are tried, plus the left edge
+    // pair) or in block br after r (blocks bl..br-1, plus the right edge
+    // pair).  The shrunk window's argmin inside [l, r] makes it the
+    // answer for the whole blocks, and the one edge pair completes it.
+    if constexpr (TWO_STAGE) {
+        if (sup.i < l) {
+            const Pair s2 = table_min(st, nb, bl + 1, br - bl);
+            if (s2.i <= r) {
+                Pair x;
+                if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
+                return (x.v <= s2.v) ? x : s2;  // the left part's indices are smaller: it wins ties
+            }
+        } else if ((int64_t)sup.i < (int64_t)(br + 1) * bs) {
+            const Pair s2 = table_min(st, nb, bl, br - bl);
+            if (s2.i >= l) {
+                Pair x;
+                if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
+                return (x.v < s2.v) ? x : s2;  // the right part's indices are larger: strictly smaller wins
+            }
+        }
+    }
+    // Whole blocks between, then each edge from its block's (min, leftmost
+    // argmin) in table row 0: exact when the argmin lies in the edge part,
+    // skipped when the block minimum cannot win, and only otherwise read
+    // from the edge table.
+    Pair best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
+    const Pair el = st[bl];
+    if (el.i >= l) {
+        if (el.v <= best.v) best = el;  // earlier indices win ties
+    } else if (el.v <= best.v) {
+        Pair x;
+        if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
+        if (x.v <= best.v) best = x;
+    }
+    const Pair er = st[br];
+    if (er.i <= r) {
+        if (er.v < best.v) best = er;  // later indices must be smaller
+    } else if (er.v < best.v) {
+        Pair x;
+        if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
+        if (x.v < best.v) best = x;
+    }
+    return best;
+}
+
+// A range inside block bl: the superset test over its 32-element sub-blocks
+// jl..jr, else the sub-block edge pairs and the sub-blocks between.
+template <bool NT_PAIRS>
+__device__ __forceinline__ Pair answer_in_block(const EdgeTables &e, const Blocks &k, int l, int r, int bl) {
+    const int base = bl * k.bs;
+    const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
+    const Pair sup = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl, jr - jl + 1);
+    if (sup.i >= l && sup.i <= r) return sup;
+    Pair best, right;
+    if constexpr (NT_PAIRS) best = nt_load_pair(e.suf_s + l); else best = e.suf_s[l];
+    if constexpr (NT_PAIRS) right = nt_load_pair(e.pre_s + r); else right = e.pre_s[r];
+    if (jr - jl > 1) {
+        const Pair mid = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl + 1, jr - jl - 1);
+        if (mid.v < best.v) best = mid;
+    }
+    if (right.v < best.v) best = right;
+    return best;
+}
+
 // O(1) answer, from the edge tables, of a range not inside one 32-element
 // sub-block: blocks differ -> suffix of the left block, whole blocks between
 // (block table), prefix of the right block; same block -> the same three
 // parts one level down (the range spans at least two sub-blocks of the
-// block).  Parts are combined left to right with strict '<', so the
-// leftmost minimum wins.  One lane per query: no reduction needed.
+// block).  Parts are combined left to right with strict '<', so the leftmost
+// minimum wins.  One lane per query: no reduction needed.  Before any part is
+// read, the minimum of a window of whole blocks covering bl..br is the answer
+// when its leftmost argmin lies in [l, r] (every element of [l, r] before it
+// is strictly larger); otherwise answer_failed.
 // LINES: the table is in global memory (base 256-byte aligned), where each
-// read costs a 128-byte line (see the superset test below).  NT_PAIRS: the
-// per-element edge pairs are loaded non-temporally (measured: uniform lengths
-// with the table in LDS 0.386 -> 0.342 ms; with the table in global memory it
-// costs the lognormal 2^14 batch 10%, so that kernel keeps cached loads).
+// read costs a 128-byte line.  NT_PAIRS: the per-element edge pairs are
+// loaded non-temporally (measured: uniform lengths with the table in LDS
+// 0.386 -> 0.342 ms; with the table in global memory it costs the lognormal
+// 2^14 batch 10%, so that kernel keeps cached loads).
 // TWO_STAGE: a failed superset test is retried with the failing side cut off
-// (below).  It saves DRAM lines where the table misses the caches (lognormal
-// 2^6 lengths at bs=8, a 2.4 GB table: 2.14 -> 2.02 ms) and costs where it
-// hits them (lognormal 2^14 at bs=2048, 6 MB: +3.5%), so the host turns it on
-// for large tables only (kTwoStageTableBytes).
+// (answer_failed).  It saves DRAM lines where the table misses the caches
+// (lognormal 2^6 lengths at bs=8, a 2.4 GB table: 2.14 -> 2.02 ms) and costs
+// where it hits them (lognormal 2^14 at bs=2048, 6 MB: +3.5%), so the host
+// turns it on for large tables only (kTwoStageTableBytes).
 template <bool LINES = true, bool NT_PAIRS = false, bool TWO_STAGE = false>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
-    const Pair *__restrict__ st = k.st;
-    const int nb = k.nb, bs = k.bs;
     const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
-    Pair best, right;
     if (bl != br) {
-        // The minimum of a window of whole blocks covering bl..br, when its
-        // leftmost argmin lies in [l, r], is the answer (every element of
-        // [l, r] before it is strictly larger).  The window is bl..br, two
-        // table reads; when those two entries lie in different 128-byte
-        // lines, it is the one level-(lev+1) entry at bl instead (blocks
-        // bl..bl+2^(lev+1)-1, up to twice as many), one line instead of two.
-        // Otherwise: whole blocks between, then each edge from its block's
-        // (min, leftmost argmin) in table row 0: exact when the argmin lies
-        // in the edge part, skipped when the block minimum cannot win, and
-        // only otherwise read from the edge table.
-        const int c = br - bl + 1;
-        const int lev = 31 - __clz(c);
-        const Pair *row = st + (int64_t)lev * nb;
-        const int a2 = br - (1 << lev) + 1;
-        Pair sup;
-        if (LINES && ((reinterpret_cast<uintptr_t>(row + bl) ^ reinterpret_cast<uintptr_t>(row + a2)) >> 7) != 0 &&
-            (int64_t)bl + ((int64_t)2 << lev) <= nb) {
-            sup = row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
-        } else {
-            const Pair x = row[bl];
-            const Pair y = row[a2];
-            sup = (y.v < x.v) ? y : x;
-        }
+        const Pair sup = superset_min<LINES>(k.st, k.nb, bl, br);
         if (sup.i >= l && sup.i <= r) return sup;
-        // Second test, with the side that failed cut off: the argmin lies in
-        // block bl before l (blocks bl+1..br are tried, plus the left edge
-        // pair) or in block br after r (blocks bl..br-1, plus the right edge
-        // pair).  The shrunk window's argmin inside [l, r] makes it the
-        // answer for the whole blocks, and the one edge pair completes it.
-        if constexpr (TWO_STAGE) {
-            if (sup.i < l) {
-                const Pair s2 = table_min(st, nb, bl + 1, br - bl);
-                if (s2.i <= r) {
-                    Pair x;
-                    if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
-                    return (x.v <= s2.v) ? x : s2;  // the left part's indices are smaller: it wins ties
-                }
-            } else if ((int64_t)sup.i < (int64_t)(br + 1) * bs) {
-                const Pair s2 = table_min(st, nb, bl, br - bl);
-                if (s2.i >= l) {
-                    Pair x;
-                    if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
-                    return (x.v < s2.v) ? x : s2;  // the right part's indices are larger: strictly smaller wins
-                }
-            }
-        }
-        best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
-        const Pair el = st[bl];
-        if (el.i >= l) {
-            if (el.v <= best.v) best = el;  // earlier indices win ties
-        } else if (el.v <= best.v) {
-            Pair x;
-            if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
-            if (x.v <= best.v) best = x;
-        }
-        const Pair er = st[br];
-        if (er.i <= r) {
-            if (er.v < best.v) best = er;  // later indices must be smaller
-        } else if (er.v < best.v) {
-            Pair x;
-            if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
-            if (x.v < best.v) best = x;
-        }
-        return best;
-    } else {
-        const int base = bl * bs;
-        const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
-        // the same superset test over the sub-blocks jl..jr of this block
-        const Pair sup = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl, jr - jl + 1);
-        if (sup.i >= l && sup.i <= r) return sup;
-        if constexpr (NT_PAIRS) best = nt_load_pair(e.suf_s + l); else best = e.suf_s[l];
-        if constexpr (NT_PAIRS) right = nt_load_pair(e.pre_s + r); else right = e.pre_s[r];
-        if (jr - jl > 1) {
-            const Pair mid = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl + 1, jr - jl - 1);
-            if (mid.v < best.v) best = mid;
-        }
+        return answer_failed<NT_PAIRS, TWO_STAGE>(e, k, l, r, bl, br, sup);
     }
-    if (right.v < best.v) best = right;
-    return best;
+    return answer_in_block<NT_PAIRS>(e, k, l, r, bl);
 }
 
 // Per-wave LDS staging of one tile's queries of a class.
