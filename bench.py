@@ -734,6 +734,55 @@ def run_served(R, torch, dev, args) -> dict:
                                                                             poi["peak_inflight_bytes"])}
 
 
+def run_stress(R, torch, dev, args) -> dict:
+    """Config 3: n=2^27 arrays of heavy ties (values in {0..15}) and strictly
+    descending values, fixed range lengths 1, 2^8, 2^16 and n-l (every range
+    ends at n-1); per workload the autotuner's pick (tuned afresh on 2^24 of
+    the queries, no cache), its RMQ/s, and the literal bs=2^15 configuration
+    beside it.  The two configurations' answers are compared in full."""
+    n, q = 1 << 27, args.stress_q
+    sample = min(q, args.tune_sample)
+    out = {}
+    v_t = torch.empty(q, dtype=torch.float32, device=dev)
+    i_t = torch.empty(q, dtype=torch.int32, device=dev)
+    v_f = torch.empty_like(v_t)
+    i_f = torch.empty_like(i_t)
+
+    def rate(ix, Q, opts, v, i):
+        ix.query(Q, opts=opts, out_val=v, out_idx=i)
+        ms = [ix.query_timed(Q, opts=opts, out_val=v, out_idx=i)[2]["total_ms"] for _ in range(args.steps)]
+        return sum(ms) / len(ms)
+
+    for kind in ("ties16", "desc"):
+        A = R.gen_array(n, 99, kind, device=dev)
+        for lr in (1, 1 << 8, 1 << 16, -6):
+            Q = R.gen_queries_device(q, n, lr, args.seed)
+            pick = R.tune(A, Q[:sample], cache_path=None)
+            with R.RMQIndex(A, pick["bs"], flags=pick["flags"]) as ix:
+                ms_t = rate(ix, Q, pick["opts"], v_t, i_t)
+                kern = ix.kernels(pick["opts"])
+            with R.RMQIndex(A, 1 << 15) as ix:
+                ms_f = rate(ix, Q, None, v_f, i_f)
+            same = bool(torch.equal(i_t, i_f) and torch.equal(v_t.view(torch.int32), v_f.view(torch.int32)))
+            o = pick["opts"]
+            name = {1: "len 1", 256: "len 2^8", 65536: "len 2^16", -6: "len n-l"}[lr]
+            out.setdefault(kind, {})[name] = {
+                "tuned": {"bs": pick["bs"], "edges": bool(pick["flags"]), "short_max": o.short_max,
+                          "group_lanes": o.group_lanes, "long_lanes": o.long_lanes, "kernels": kern,
+                          "ms_per_pass": ms_t, "rmq_per_s": q / (ms_t * 1e-3)},
+                "bs_32768": {"ms_per_pass": ms_f, "rmq_per_s": q / (ms_f * 1e-3)},
+                "answers_identical": same}
+            log(f"stress {kind:6s} {name:9s} tuned bs={pick['bs']} edges={bool(pick['flags'])} "
+                f"s{o.short_max}/g{o.group_lanes}/l{o.long_lanes} {q / (ms_t * 1e-3):.3e} RMQ/s; "
+                f"bs=2^15 {q / (ms_f * 1e-3):.3e}; identical={same}")
+            if not same:
+                raise SystemExit(f"stress {kind} {name}: tuned and bs=2^15 answers differ")
+            del Q
+        del A
+        torch.cuda.empty_cache()
+    return {"n": n, "q": q, "tune_sample": sample, "workloads": out}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -762,7 +811,8 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered rate / saturation rate")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--only", choices=["batched", "served"], default=None,
+    ap.add_argument("--stress-q", type=int, default=1 << 26, help="queries per workload of --only stress")
+    ap.add_argument("--only", choices=["batched", "served", "stress"], default=None,
                     help="run one section alone and print its JSON object (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
@@ -797,7 +847,7 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     if args.only:
-        res = run_batched(R, torch, dev, args) if args.only == "batched" else run_served(R, torch, dev, args)
+        res = {"batched": run_batched, "served": run_served, "stress": run_stress}[args.only](R, torch, dev, args)
         print(json.dumps({args.only: res}), flush=True)
         return 0
     n, q = args.n, args.q
