@@ -92,12 +92,14 @@ class BlockLevel:
         t = yv < xv
         return torch.where(t, yv, xv), torch.where(t, yi, xi)
 
-    def superset(self, lB, rB, lines: bool):
+    def superset(self, lB, rB, lines: bool, retry: bool = False):
         """(value, index, table entries read) of the superset test of blocks
         lB < rB: the minimum of lB..rB from two level-lev entries, or, with
         lines (a global table, 256-byte aligned), when those two lie in
         different 128-byte lines and the window fits, the one level-(lev+1)
-        entry at lB (blocks lB .. lB + 2^(lev+1) - 1)."""
+        entry at lB (blocks lB .. lB + 2^(lev+1) - 1); with retry, when that
+        entry's argmin lies past block rB, the two level-lev entries after it
+        (three reads)."""
         import torch
         c = rB - lB + 1
         v, i = self.range_min(lB, c)
@@ -108,10 +110,11 @@ class BlockLevel:
             a2 = rB - torch.bitwise_left_shift(torch.ones_like(lev), lev) + 1
             split = ((base + lB) * 8) // 128 != ((base + a2) * 8) // 128
             one = split & (lB + torch.bitwise_left_shift(torch.full_like(lev, 2), lev) <= self.nb)
-            if bool(one.any()):
-                v[one] = self.V[lev[one] + 1, lB[one]]
-                i[one] = self.I[lev[one] + 1, lB[one]]
-            reads = torch.where(one, 1, 2)
+            lev1 = torch.where(one, lev + 1, lev)  # level lev+1 exists where one holds
+            wv, wi = self.V[lev1, lB], self.I[lev1, lB]
+            use = one & (wi < (rB + 1) * self.bs) if retry else one
+            v, i = torch.where(use, wv, v), torch.where(use, wi, i)
+            reads = torch.where(use, 1, torch.where(one, 3, 2))
         return v, i, reads
 
 
@@ -184,11 +187,12 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     between.  sub: the 32-element level and A the array, with edge tables.
     lines: the superset test reads one level-up entry instead of two entries
     in different 128-byte lines (BlockLevel.superset), as the edge-table
-    kernel with its table in global memory does."""
+    kernel with its table in global memory does; with two_stage as well, an
+    argmin of that entry past block rB is followed by the two entries."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
-    n_long = n_short = parts = needed = gathers = reads = lines = 0
+    n_long = n_short = parts = needed = gathers = reads = short_lines = 0
     for c0 in range(0, Q.shape[0], 1 << 23):
         l = Q[c0:c0 + (1 << 23), 0].long()
         r = Q[c0:c0 + (1 << 23), 1].long()
@@ -199,7 +203,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             is_long &= ~((r // bs * bs == base) & ((l - base) // 32 == (r - base) // 32))
         short_b += float((8 + 4 * ln[~is_long] + out_bytes).sum())
         n_short += int((~is_long).sum())
-        lines += int((r[~is_long] // 32 - l[~is_long] // 32 + 1).sum())
+        short_lines += int((r[~is_long] // 32 - l[~is_long] // 32 + 1).sum())
         l, r, ln = l[is_long], r[is_long], ln[is_long]
         n_long += int(l.numel())
         lB, rB = l // bs, r // bs
@@ -216,7 +220,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             long_b += float((8 + scan_bytes(l[same], r[same], bs) + out_bytes).sum())
         x = ~same
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
-        _, sup_i, sr = lvl.superset(lB, rB, edges and lines)
+        _, sup_i, sr = lvl.superset(lB, rB, edges and lines, retry=edges and lines and two_stage)
         fast = (sup_i >= l) & (sup_i <= r)
         long_b += float((8 + 8 * sr[fast] + out_bytes).sum())
         parts += 2 * int(fast.sum())
@@ -267,7 +271,7 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         long_b += float((8 + 8 * sr + s2b + 16 * mid.long() + 16 + edge_b + out_bytes).sum())
     return {"long": long_b, "short": short_b, "n_long": n_long, "n_short": n_short,
             "edge_parts_computed": needed / parts if parts else 0.0, "gathers": gathers, "table_reads": reads,
-            "short_lines": lines}
+            "short_lines": short_lines}
 
 
 XCDS = 8  # accelerator complex dies of an MI355X (GRBM_GUI_ACTIVE counts each)

@@ -381,15 +381,22 @@ struct EdgeTables {
 // bl..br from two level-lev entries, or, with LINES, when those two entries
 // lie in different 128-byte lines, the one level-(lev+1) entry at bl (blocks
 // bl..bl+2^(lev+1)-1, up to twice as many): one line instead of two.
-template <bool LINES>
-__device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb, int bl, int br) {
+// RETRY: when that wider window's argmin lies past block br, it says nothing
+// about bl..br, and the two level-lev entries are read after all (a CPU
+// replay of the lognormal 2^6 batch at bs=8: 5% of ranges, which the
+// failure path would otherwise take; 1.40 -> 1.32 DRAM lines per range).  On
+// a table in L2 the extra dependent read costs more than it saves
+// (lognormal 2^14 at bs=1024: 0.93 -> 0.98 ms), so it goes with TWO_STAGE.
+template <bool LINES, bool RETRY>
+__device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb, int bs, int bl, int br) {
     const int c = br - bl + 1;
     const int lev = 31 - __clz(c);
     const Pair *row = st + (int64_t)lev * nb;
     const int a2 = br - (1 << lev) + 1;
     if (LINES && ((reinterpret_cast<uintptr_t>(row + bl) ^ reinterpret_cast<uintptr_t>(row + a2)) >> 7) != 0 &&
         (int64_t)bl + ((int64_t)2 << lev) <= nb) {
-        return row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
+        const Pair w = row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
+        if (!RETRY || (int64_t)w.i < (int64_t)(br + 1) * bs) return w;
     }
     const Pair x = row[bl];
     const Pair y = row[a2];
@@ -491,7 +498,7 @@ template <bool LINES = true, bool NT_PAIRS = false, bool TWO_STAGE = false>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
     const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
     if (bl != br) {
-        const Pair sup = superset_min<LINES>(k.st, k.nb, bl, br);
+        const Pair sup = superset_min<LINES, TWO_STAGE>(k.st, k.nb, k.bs, bl, br);
         if (sup.i >= l && sup.i <= r) return sup;
         return answer_failed<NT_PAIRS, TWO_STAGE>(e, k, l, r, bl, br, sup);
     }

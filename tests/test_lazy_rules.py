@@ -70,7 +70,9 @@ def blocks_min(bm, lo, hi):
 
 
 def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False):
-    """(answer, parts computed, superset hit) for lB < rB, following rules
+    """(answer, parts computed, superset hit, sparse-table entries read) for
+    lB < rB (a window's minimum costs two entries, a widened window one),
+    following rules
     1-3.  left_counts: a computed left part joins the best so far before the
     right part is decided (the edge-table path reads it as one pair); False
     for the scan path, whose left scan is spread over a lane group.
@@ -78,24 +80,32 @@ def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False):
     off -- argmin before l: blocks lB+1..rB, and if that argmin is <= r the
     answer is it and the left part; argmin in block rB after r: blocks
     lB..rB-1 and the right part (one part computed) -- as the edge-table
-    kernels do."""
+    kernels do; and a widened window (lines) whose argmin lies past block rB
+    is replaced by the exact window lB..rB first."""
     lB, rB = l // bs, r // bs
     lo, hi = superset_window(len(bm), lB, rB, lines)
     sup = blocks_min(bm, lo, hi)
+    reads = 1 if hi > rB else 2
+    if two_stage and hi > rB and sup[1] >= (rB + 1) * bs:
+        sup = blocks_min(bm, lB, rB)  # the widened window's argmin lies past rB: the exact window
+        reads += 2
     if l <= sup[1] <= r:
-        return sup, 0, True
+        return sup, 0, True, reads
     if two_stage:
         if sup[1] < l:
             s2 = blocks_min(bm, lB + 1, rB)
+            reads += 2
             if s2[1] <= r:
-                return lexmin(brute(A, l, lB * bs + bs - 1), s2), 1, False
+                return lexmin(brute(A, l, lB * bs + bs - 1), s2), 1, False, reads
         elif sup[1] < (rB + 1) * bs:
             s2 = blocks_min(bm, lB, rB - 1)
+            reads += 2
             if s2[1] >= l:
-                return lexmin(s2, brute(A, rB * bs, r)), 1, False
+                return lexmin(s2, brute(A, rB * bs, r)), 1, False, reads
     best = (INF, IMAX)
     for b in range(lB + 1, rB):
         best = lexmin(bm[b], best)
+    reads += (2 if rB - lB > 1 else 0) + 2  # whole blocks between; the two edge blocks' entries
     need = 0
     left = (INF, IMAX)
     el = bm[lB]
@@ -112,7 +122,7 @@ def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False):
     elif er[0] < best[0]:
         best = lexmin(brute(A, rB * bs, r), best)
         need += 1
-    return lexmin(left, best), need, False
+    return lexmin(left, best), need, False, reads
 
 
 def arrays(rng):
@@ -187,7 +197,7 @@ def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
             Q.append((l, r))
         Q = np.array(Q, np.int32)
         bm = block_mins(A, bs)
-        parts = need = gathers = n_short = 0
+        parts = need = gathers = n_short = reads = 0
         for l, r in Q:
             lB, rB = l // bs, r // bs
             one_sub = lB == rB and (l - lB * bs) // 32 == (r - lB * bs) // 32
@@ -200,11 +210,14 @@ def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
                     lo = base + (l - base) // 32 * 32
                     hi = min(base + (r - base) // 32 * 32 + 31, min(base + bs, n) - 1)
                     s = brute(A, lo, hi)
-                    gathers += 0 if l <= s[1] <= r else 2
+                    hit = l <= s[1] <= r
+                    gathers += 0 if hit else 2
+                    reads += 2 + (0 if hit or (r - base) // 32 - (l - base) // 32 <= 1 else 2)
                 continue
-            _, k, _ = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines, two_stage=two_stage)
+            _, k, _, rd = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines, two_stage=two_stage)
             parts += 2
             need += k
+            reads += rd
             gathers += k if edges else 0
         lvl = bench.BlockLevel(torch.from_numpy(A), bs)
         sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
@@ -213,5 +226,6 @@ def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
         assert ab["n_short"] == n_short
         assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
         assert ab["gathers"] == gathers
+        assert ab["table_reads"] == reads
         for k in range(lvl.nb):  # block minima: leftmost argmin, all-+inf block -> its start
             assert int(lvl.I[0, k]) == bm[k][1]
