@@ -81,7 +81,8 @@ void usage() {
             "  --seed <s>            generator seed (default time(0))\n"
             "  --check               compare with the GPU brute force (alg 2/102), bit for bit\n"
             "  --save-time[=file]    append a CSV row (default ../results/data.csv)\n"
-            "  --save-power[=file]   sample GPU power during the query loop\n"
+            "  --save-power[=file]   sample power during the query loop: the GPU's (amdgpu hwmon, 100 ms)\n"
+            "                        for GPU algs, the CPU packages' (powercap RAPL, 500 ms) for CPU algs\n"
             "  --short-max <len>     ranges up to this length use the group scan (alg 5/105)\n"
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
@@ -358,14 +359,21 @@ int main(int argc, char **argv) {
     std::vector<int32_t> out_i(idx_out ? a.q : 0);
     double time_ms = 0, construction_ms = 0, out_bytes = 0, temp_bytes = 0;
     PowerSampler power;
+    CpuEnergySampler cpu_power;
     if (cpu_alg) {
         printf(C_CYAN "Computing RMQs%s (%-16s,nt=%2i,r=%-3i).." C_RESET, idx_out ? " index" : "", alg_name(a.alg), a.nt,
                a.reps);
         fflush(stdout);
+        if (a.save_power) cpu_power.start(a.power_file, alg_name(a.alg));
         t0 = now_ms();
         for (int r = 0; r < a.reps; ++r)
             cpu_base(a, hA.data(), hQ.data(), idx_out ? nullptr : out_v.data(), idx_out ? out_i.data() : nullptr);
         time_ms = now_ms() - t0;
+        if (a.save_power) {
+            cpu_power.stop();
+            if (cpu_power.energy_j() > 0)
+                printf("energy efficiency: %.4e RMQ/J\n", (double)a.q * a.reps / cpu_power.energy_j());
+        }
         print_rate("", a, time_ms);
     } else {
         float *d_val = nullptr;

@@ -130,3 +130,56 @@ def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
     assert "power:" in r.stdout
     m = re.search(r"energy efficiency: ([0-9.e+]+) RMQ/J", r.stdout)  # q * reps / J
     assert m and float(m.group(1)) > 0
+
+
+def test_save_power_cpu_alg_reads_powercap_packages(tmp_path):
+    """CPU algs sample the CPU packages' energy every 500 ms, as the reference
+    samples RAPL around its CPU methods (common/nvmlPower.cpp:216-274), into
+    the same six columns.  A fake powercap tree: a package zone whose counter
+    a thread advances by 0.5 J every 50 ms and wraps at 30 J (after 3 s), and a non-package
+    zone that must be ignored."""
+    import threading
+    import time
+
+    root = tmp_path / "powercap"
+    pkg, core = root / "intel-rapl:0", root / "intel-rapl:0:0"
+    for z, name in ((pkg, "package-0"), (core, "core")):
+        z.mkdir(parents=True)
+        (z / "name").write_text(name + "\n")
+        (z / "energy_uj").write_text("0\n")
+        (z / "max_energy_range_uj").write_text("30000000\n")
+    stop = threading.Event()
+
+    def tick():
+        e = 0
+        while not stop.is_set():
+            time.sleep(0.05)
+            e = (e + 500_000) % 30_000_000
+            tmp = pkg / "energy_uj.tmp"
+            tmp.write_text(f"{e}\n")
+            os.replace(tmp, pkg / "energy_uj")
+            (core / "energy_uj").write_text(f"{e * 7}\n")
+
+    th = threading.Thread(target=tick)
+    th.start()
+    pfile = tmp_path / "cpu.dat"
+    try:
+        env = dict(os.environ, RTXRMQ_POWERCAP=str(root))
+        r = subprocess.run([CLI, "200000", "50000", "-2", "0", "--nt", "2", "--reps", "5", "--seed", "3",
+                            f"--save-power={pfile}"], capture_output=True, text=True, timeout=300, env=env)
+    finally:
+        stop.set()
+        th.join()
+    out = ANSI.sub("", r.stdout)
+    assert r.returncode == 0, r.stderr[-1500:]
+    rows = [ln.split() for ln in pfile.read_text().splitlines() if ln.strip() and not ln.startswith("#")]
+    assert len(rows) >= 4 and all(len(x) == 6 for x in rows)
+    vals = np.array(rows, dtype=np.float64)
+    assert vals[-1, 5] > 3.0  # the counter wrapped at least once meanwhile
+    assert np.all(np.diff(vals[:, 2]) >= 0) and np.all(vals[:, 1] >= 0)
+    # about 10 J/s from the one package zone (the core zone ignored), wraps included
+    rate = vals[-1, 2] / vals[-1, 5]
+    assert 5.0 < rate < 15.0, (rate, vals[-3:])
+    assert "CPU power:" in out and "(1 package)" in out
+    m = re.search(r"energy efficiency: ([0-9.e+]+) RMQ/J", out)
+    assert m and float(m.group(1)) > 0
