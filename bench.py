@@ -742,8 +742,9 @@ def run_stress(R, torch, dev, args) -> dict:
     """Config 3: n=2^27 arrays of heavy ties (values in {0..15}) and strictly
     descending values, fixed range lengths 1, 2^8, 2^16 and n-l (every range
     ends at n-1); per workload the autotuner's pick (tuned afresh on 2^24 of
-    the queries, no cache), its RMQ/s, and the literal bs=2^15 configuration
-    beside it.  The two configurations' answers are compared in full."""
+    the queries, no cache) and its RMQ/s, and the literal method (no edge
+    tables, default lanes) swept over bs = 2^10 .. 2^18.  Every swept
+    configuration's answers are compared in full with the tuned ones."""
     n, q = 1 << 27, args.stress_q
     sample = min(q, args.tune_sample)
     out = {}
@@ -765,22 +766,26 @@ def run_stress(R, torch, dev, args) -> dict:
             with R.RMQIndex(A, pick["bs"], flags=pick["flags"]) as ix:
                 ms_t = rate(ix, Q, pick["opts"], v_t, i_t)
                 kern = ix.kernels(pick["opts"])
-            with R.RMQIndex(A, 1 << 15) as ix:
-                ms_f = rate(ix, Q, None, v_f, i_f)
-            same = bool(torch.equal(i_t, i_f) and torch.equal(v_t.view(torch.int32), v_f.view(torch.int32)))
+            sweep, same = {}, True
+            for lg in range(10, 19):  # the literal method (no edge tables, default lanes) at bs = 2^10 .. 2^18
+                with R.RMQIndex(A, 1 << lg) as ix:
+                    ms_f = rate(ix, Q, None, v_f, i_f)
+                sweep[f"2^{lg}"] = q / (ms_f * 1e-3)
+                same &= bool(torch.equal(i_t, i_f) and torch.equal(v_t.view(torch.int32), v_f.view(torch.int32)))
             o = pick["opts"]
             name = {1: "len 1", 256: "len 2^8", 65536: "len 2^16", -6: "len n-l"}[lr]
             out.setdefault(kind, {})[name] = {
                 "tuned": {"bs": pick["bs"], "edges": bool(pick["flags"]), "short_max": o.short_max,
                           "group_lanes": o.group_lanes, "long_lanes": o.long_lanes, "kernels": kern,
                           "ms_per_pass": ms_t, "rmq_per_s": q / (ms_t * 1e-3)},
-                "bs_32768": {"ms_per_pass": ms_f, "rmq_per_s": q / (ms_f * 1e-3)},
+                "bs_sweep_rmq_per_s": sweep,
                 "answers_identical": same}
+            best = max(sweep, key=sweep.get)
             log(f"stress {kind:6s} {name:9s} tuned bs={pick['bs']} edges={bool(pick['flags'])} "
                 f"s{o.short_max}/g{o.group_lanes}/l{o.long_lanes} {q / (ms_t * 1e-3):.3e} RMQ/s; "
-                f"bs=2^15 {q / (ms_f * 1e-3):.3e}; identical={same}")
+                f"sweep bs=2^15 {sweep['2^15']:.3e}, best bs={best} {sweep[best]:.3e}; identical={same}")
             if not same:
-                raise SystemExit(f"stress {kind} {name}: tuned and bs=2^15 answers differ")
+                raise SystemExit(f"stress {kind} {name}: tuned and swept answers differ")
             del Q
         del A
         torch.cuda.empty_cache()
