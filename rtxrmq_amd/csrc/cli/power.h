@@ -39,7 +39,10 @@ class PowerSampler {
             path += ".dat";
         }
         out_ = fopen(path.c_str(), "w+");
-        if (!out_) return false;
+        if (!out_) {
+            fprintf(stderr, "--save-power: cannot open %s; skipping\n", path.c_str());
+            return false;
+        }
         fprintf(out_, "%-15s %-15s %-15s %-15s %-15s %-15s\n", "#timestep", "power", "acc-energy", "avg-power", "dt",
                 "acc-time");
         running_ = true;
@@ -141,7 +144,10 @@ class CpuEnergySampler {
             path += ".dat";
         }
         out_ = fopen(path.c_str(), "w+");
-        if (!out_) return false;
+        if (!out_) {
+            fprintf(stderr, "--save-power: cannot open %s; skipping\n", path.c_str());
+            return false;
+        }
         fprintf(out_, "%-15s %-15s %-15s %-15s %-15s %-15s\n", "#timestep", "power", "acc-energy", "avg-power", "dt",
                 "acc-time");
         for (Zone &z : zones_) z.last = read_uj(z.energy);
