@@ -119,21 +119,25 @@ class BlockLevel:
 
 
 HIER_MIN = 4  # kHierMin: whole sub-blocks between the ends before a scan reads their minima
+HIER_TABLE = 16  # kHierTable: more whole sub-blocks than this are read as two sparse-table entries
 
 
 def scan_bytes(lo, hi, bs: int):
     """Bytes of the scan of [lo, hi] inside one block: element by element, or,
     with at least HIER_MIN whole 32-element sub-blocks between the ends (and
     bs > 32), the partial sub-blocks element by element and the whole ones as
-    8-byte (min, argmin) pairs."""
+    8-byte (min, argmin) pairs, or, beyond HIER_TABLE of them, as two entries
+    (16 B) of the sparse table over sub-block minima."""
     import torch
     direct = 4 * (hi - lo + 1)
     if bs <= 32:
         return direct
     base = lo // bs * bs
     jl, jr = (lo - base) // 32, (hi - base) // 32
-    hier = 4 * (base + jl * 32 + 32 - lo) + 8 * (jr - jl - 1) + 4 * (hi - (base + jr * 32) + 1)
-    return torch.where(jr - jl - 1 >= HIER_MIN, hier, direct)
+    whole = jr - jl - 1
+    mid = torch.where(whole > HIER_TABLE, torch.full_like(whole, 16), 8 * whole)
+    hier = 4 * (base + jl * 32 + 32 - lo) + mid + 4 * (hi - (base + jr * 32) + 1)
+    return torch.where(whole >= HIER_MIN, hier, direct)
 
 
 def left_part_min(A, sub, bs: int, l):
@@ -802,8 +806,9 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=1 << 26)
     ap.add_argument("--bs", type=int, default=1 << 15, help="block size of the fixed-bs run")
     ap.add_argument("--fixed-steps", type=int, default=2, help="timed steps of the fixed-bs run")
-    ap.add_argument("--tune-sample", type=int, default=1 << 24,
-                    help="queries per batch the autotuner times (2^22 gave noisier picks, 6%% slower)")
+    ap.add_argument("--tune-sample", type=int, default=1 << 26,
+                    help="queries per batch the autotuner times (the whole batch by default: picks made on "
+                         "2^22 queries ran 6%% slower at full size, on 2^24 up to 3%%)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "rtxrmq_amd", "tune_cache_mi355x.txt"),
                     help="autotuner cache (picks per n and length signature); '' for none")
     ap.add_argument("--retune", action="store_true", help="time the sweep even when the cache has a pick")

@@ -29,9 +29,16 @@ constexpr int64_t kTwoStageTableBytes = int64_t(64) << 20;  // block tables beyo
 constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
 constexpr int kDefaultShortMax = 1024;
 constexpr int kDefaultGroupLanes = 16;
+// Lanes per long query by default: with the sparse table over sub-block
+// minima a long in-block scan is two partial sub-blocks and two entries, so
+// more queries in flight beat more lanes per query (literal bs=2^15,
+// lognormal 2^14 lengths: 64 lanes 29.4 ms, 16 lanes 13.4, 4 lanes 6.5;
+// uniform lengths 1.01 -> 1.07 ms).
+constexpr int kDefaultLongLanes = 4;
 constexpr int kSub = 32;  // sub-block size (elements); sub-blocks restart at every block start
 constexpr int kSubShift = 5;
 constexpr int kHierMin = 4;  // whole sub-blocks between the ends before a scan reads their minima instead
+constexpr int kHierTable = 16;  // more whole sub-blocks than this: two sparse-table entries instead of the minima
 
 struct __align__(8) Pair {
     float v;
@@ -233,7 +240,8 @@ __device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t s
 struct Blocks {
     const float *A;
     const Pair *st;   // levels rows of nb pairs; row 0 = each block's (min, leftmost argmin)
-    const Pair *sub;  // nb * nsub pairs: each 32-element sub-block's (min, leftmost argmin); null when bs <= 32
+    const Pair *sub;  // sparse table over the nb * nsub sub-block minima (row 0: each 32-element sub-block's (min,
+                      // leftmost argmin)); null when bs <= 32
     uint64_t magic;   // block_magic(bs)
     int n, bs, nb, nsub;
 };
@@ -293,17 +301,24 @@ __device__ __forceinline__ void scan_pairs(const Pair *__restrict__ P, int64_t a
 // Scans [lo, hi] inside block b.  With a sub-block level and at least
 // kHierMin whole sub-blocks between the ends, the partial sub-blocks at both
 // ends are scanned element by element and the whole ones between read as
-// their (min, argmin) pairs: 8 bytes per 32 elements.  Otherwise one direct
-// scan.
+// their (min, argmin) pairs, 8 bytes per 32 elements, or, beyond kHierTable
+// of them, as two entries of the sparse table over sub-block minima (its
+// rows follow row 0 at a stride of nb * nsub).  Otherwise one direct scan.
 template <int G, int U>
 __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi, int gl, float &bv, int &bi) {
     if (k.sub) {
         const int base = b * k.bs;
         const int jl = (lo - base) >> kSubShift, jr = (hi - base) >> kSubShift;
-        if (jr - jl - 1 >= kHierMin) {
+        const int whole = jr - jl - 1;
+        if (whole >= kHierMin) {
             const int64_t row = (int64_t)b * k.nsub;
             scan_range<G, U>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
-            scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
+            if (whole > kHierTable) {
+                const Pair m = table_min(k.sub, (int64_t)k.nb * k.nsub, row + jl + 1, whole);
+                take_if_better(m.v, m.i, bv, bi);
+            } else {
+                scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
+            }
             scan_range<G, U>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
             return;
         }
@@ -653,7 +668,7 @@ inline void fill_default_opts(const rmq_handle *h, rmq_query_opts_t *o) {
     o->short_max = kDefaultShortMax;
     o->group_lanes = kDefaultGroupLanes;
     o->long_wgs = 0;
-    o->long_lanes = kWave;
+    o->long_lanes = kDefaultLongLanes;
 }
 
 inline int resolve_opts(const rmq_handle *h, const rmq_query_opts_t *in, rmq_query_opts_t *o) {
