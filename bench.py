@@ -808,7 +808,7 @@ def main() -> int:
     ap.add_argument("--fixed-steps", type=int, default=2, help="timed steps of the fixed-bs run")
     ap.add_argument("--tune-sample", type=int, default=1 << 26,
                     help="queries per batch the autotuner times (the whole batch by default: picks made on "
-                         "2^22 queries ran 6%% slower at full size, on 2^24 up to 3%%)")
+                         "2^22 queries ran 6%% slower at full size)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "rtxrmq_amd", "tune_cache_mi355x.txt"),
                     help="autotuner cache (picks per n and length signature); '' for none")
     ap.add_argument("--retune", action="store_true", help="time the sweep even when the cache has a pick")

@@ -984,14 +984,35 @@ void cache_store(const char *path, const char *key, const rmq_tune_result_t &r) 
     fclose(f);
 }
 
-// Best of `reps` timed passes (after one warm-up pass), milliseconds.
+// A device buffer larger than the last-level cache, written before each
+// timed pass so that the pass starts from caches holding other data, as a
+// batch does when other work ran before it.  Timed back to back on warm
+// caches, the lognormal 2^6 batch's pick ran 1.82 ms in the tuner and 1.90 ms
+// in a mixed stream; flushed, 1.88 against 1.91.
+struct CacheFlush {
+    void *buf = nullptr;
+    size_t bytes = 0;
+    explicit CacheFlush(size_t want) {
+        if (hipMalloc(&buf, want) == hipSuccess) bytes = want;
+        else (void)hipGetLastError();  // no room: time on warm caches
+    }
+    ~CacheFlush() {
+        if (buf) (void)hipFree(buf);
+    }
+    int run(hipStream_t s) const { return buf ? hip_rc(hipMemsetAsync(buf, 0, bytes, s)) : RMQ_OK; }
+};
+constexpr size_t kFlushBytes = size_t(512) << 20;  // twice the 256 MB last-level cache
+
+// Best of `reps` timed passes (after one warm-up pass), milliseconds; each
+// timed pass follows a cache flush (outside the timed events).
 int time_pass(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, int32_t *d_idx,
-              const rmq_query_opts_t &o, hipStream_t s, int reps, float *best) {
+              const rmq_query_opts_t &o, hipStream_t s, int reps, float *best, const CacheFlush *flush = nullptr) {
     rmq_query_times_t t;
     int rc = query_impl(h, d_Q, q, d_val, d_idx, &o, s, &t);
     *best = 1e30f;
     for (int k = 0; rc == RMQ_OK && k < reps; ++k) {
-        rc = query_impl(h, d_Q, q, d_val, d_idx, &o, s, &t);
+        if (flush) rc = flush->run(s);
+        if (rc == RMQ_OK) rc = query_impl(h, d_Q, q, d_val, d_idx, &o, s, &t);
         if (rc == RMQ_OK) *best = std::min(*best, t.total_ms);
     }
     return rc;
@@ -1014,6 +1035,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         out->n_tried = 0;
         return RMQ_OK;
     }
+    const CacheFlush flush(kFlushBytes);
     std::vector<int32_t> cands;
     if (n_bs > 0) {
         cands.assign(bs_list, bs_list + n_bs);
@@ -1055,7 +1077,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
                 rmq_query_opts_t o = base_opts(bs);
                 o.short_max = sm;
                 float t = 0.f;
-                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &t)) != RMQ_OK) break;
+                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &t, &flush)) != RMQ_OK) break;
                 ++tried;
                 ms = std::min(ms, t);
             }
@@ -1087,7 +1109,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         std::vector<Timed> seen;  // every option set timed for this finalist
         rmq_query_opts_t cand_o = base_opts(c.bs);
         float cand_ms = 1e30f;
-        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, cand_o, s, kReps, &cand_ms)) == RMQ_OK) {
+        if ((rc = time_pass(h, d_Q, q, d_val, d_idx, cand_o, s, kReps, &cand_ms, &flush)) == RMQ_OK) {
             ++tried;
             seen.push_back(Timed{cand_ms, cand_o});
         }
@@ -1099,7 +1121,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
                 o.short_max = sm;
                 o.group_lanes = g;
                 float ms = 0.f;
-                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms)) != RMQ_OK) break;
+                if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms, &flush)) != RMQ_OK) break;
                 ++tried;
                 seen.push_back(Timed{ms, o});
                 if (ms < cand_ms) {
@@ -1113,7 +1135,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             rmq_query_opts_t o = cand_o;
             o.long_lanes = g;
             float ms = 0.f;
-            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms)) != RMQ_OK) break;
+            if ((rc = time_pass(h, d_Q, q, d_val, d_idx, o, s, kReps, &ms, &flush)) != RMQ_OK) break;
             ++tried;
             seen.push_back(Timed{ms, o});
             if (ms < cand_ms) {
@@ -1121,10 +1143,9 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
                 cand_o = o;
             }
         }
-        // Runoff: near-equal option sets swap places between single timings
-        // (lanes 4 vs 8 for lognormal 2^6 lengths at bs=8 cost 3% at full
-        // size), so the fastest few are timed again, interleaved, and the
-        // lowest total wins.
+        // Runoff: near-equal option sets swap places between single timings,
+        // so the fastest few are timed again, interleaved, and the lowest
+        // total wins.
         if (rc == RMQ_OK) {
             std::sort(seen.begin(), seen.end(), [](const Timed &x, const Timed &y) { return x.ms < y.ms; });
             std::vector<Timed> top;
@@ -1136,7 +1157,7 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             for (int round = 0; rc == RMQ_OK && round < kRounds; ++round) {
                 for (Timed &t : top) {
                     float ms = 0.f;
-                    if ((rc = time_pass(h, d_Q, q, d_val, d_idx, t.o, s, kReps, &ms)) != RMQ_OK) break;
+                    if ((rc = time_pass(h, d_Q, q, d_val, d_idx, t.o, s, kReps, &ms, &flush)) != RMQ_OK) break;
                     ++tried;
                     t.ms += ms;
                 }
