@@ -891,7 +891,9 @@ def main() -> int:
     # --- autotuned configuration per distribution ----------------------------------
     picks = {}
     for lr in DISTS:
-        sample = Q_dev[lr][: min(q, args.tune_sample)]
+        # the tuner times a stream of the same distribution with another seed,
+        # so the picks are not fitted to the very queries that are timed
+        sample = R.gen_queries_device(min(q, args.tune_sample), n, lr, args.seed + 7919 + 17 * (-lr), device=dev)
         t0 = time.perf_counter()
         picks[lr] = R.tune(A, sample, cache_path=args.tune_cache or None, refresh=args.retune)
         p = picks[lr]

@@ -120,6 +120,13 @@ void rmq_free(rmq_handle_t *h);
 int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int64_t *nb, int32_t *levels,
                     int64_t *bytes);
 
+/* Device bytes rmq_build(n, bs, flags) allocates (what rmq_handle_info
+ * reports for the built handle), computed on the host without building.
+ * The serving loop admits new arrays by this figure.  No counterpart in the
+ * reference, which reports its BVH's output and temp buffer sizes only after
+ * building (src/rtx_functions.h:289-379, src/tools.h:316-336). */
+int rmq_handle_bytes(int64_t n, int32_t bs, uint32_t flags, int64_t *bytes);
+
 /* Names of the kernels rmq_query launches on this handle with these options
  * (NULL = defaults), separated by ';' in launch order, as a profiler reports
  * them: "k_query_edges_lds<G>" (edge tables, block sparse table held in
@@ -151,10 +158,14 @@ typedef struct rmq_tune_result {
  *      2^18), with and without edge tables (RMQ_BUILD_EDGES), then, for the
  *      four fastest of those, the short-range threshold and lanes per query,
  *      and returns the fastest configuration (build it with rmq_build(..., bs,
- *      build_flags, ...)).  Results are cached
- *      per (n, length-distribution signature) in the text file cache_path
- *      (NULL: no cache).  d_val / d_idx are scratch outputs of q entries (one
- *      may be NULL).  Synchronising; builds and frees its own handles. ---- */
+ *      build_flags, ...)).  Results are cached in the text file cache_path
+ *      (NULL: no cache) under a key of: the tuner's version, log2 n and the
+ *      10/50/90th percentiles of log2(range length) of the sample, the
+ *      array's value signature (mean position of the leftmost minimum in
+ *      sampled 64-element windows, and a tie-density class) and the outputs
+ *      timed ("vi", "v" or "i").  d_val / d_idx are scratch outputs of q
+ *      entries (one may be NULL: pass the outputs the real queries will
+ *      ask for).  Synchronising; builds and frees its own handles. ---- */
 int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,
              float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s, rmq_tune_result_t *out);
 

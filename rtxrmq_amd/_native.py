@@ -86,6 +86,7 @@ def _declare(lib):
         "rmq_free": (None, [_vp]),
         "rmq_handle_info": (ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32), ctypes.POINTER(_i64),
                                            ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
+        "rmq_handle_bytes": (ctypes.c_int, [_i64, _i32, _u32, ctypes.POINTER(_i64)]),
         "rmq_query_opts_default": (ctypes.c_int, [_vp, ctypes.POINTER(QueryOpts)]),
         "rmq_query_kernels": (ctypes.c_int, [_vp, ctypes.POINTER(QueryOpts), ctypes.c_char_p, _i64]),
         "rmq_gpu_base": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),

@@ -375,6 +375,7 @@ struct rmq_server {
     rmq_server_config_t cfg{};
     int device = -1;
     std::vector<rmq_handle *> arrays;
+    hipMemPool_t pool = nullptr;  // staging buffers of pageable requests (own pool, freed memory kept)
     ArrayDesc *d_descs = nullptr;
     int32_t descs_cap = 0;
     PlanKernel kernel = nullptr;
@@ -422,7 +423,7 @@ int launch_group(rmq_server *sv, Slot &sl, std::vector<Request *> &group) {
     }
     int rc = hip_rc(hipEventRecord(sl.start_ev, sl.stream));
     // staged requests (pageable host buffers) share one stream-ordered buffer
-    if (rc == RMQ_OK && total > 0) rc = hip_rc(hipMallocAsync(&sl.d_buf, (size_t)total, sl.stream));
+    if (rc == RMQ_OK && total > 0) rc = hip_rc(hipMallocFromPoolAsync(&sl.d_buf, (size_t)total, sv->pool, sl.stream));
     if (rc != RMQ_OK) return rc;
     char *p = static_cast<char *>(sl.d_buf);
     size_t k = 0;
@@ -628,6 +629,7 @@ void server_release(rmq_server *sv) {
     }
     for (rmq_handle *h : sv->arrays) rmq_free(h);
     if (sv->d_descs) (void)hipFree(sv->d_descs);
+    if (sv->pool) (void)hipMemPoolDestroy(sv->pool);  // every staging buffer was freed on a synchronised stream
     delete sv;
 }
 
@@ -650,12 +652,18 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
     if (rc == RMQ_OK) rc = resolve_opts(&dummy, &c.opts, &c.opts);
     sv->cfg = c;
     if (rc == RMQ_OK) {
-        // request buffers come from the device's stream-ordered pool; keep what
-        // is freed cached instead of returning it at every synchronisation
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, sv->device) == hipSuccess) {
+        // staging buffers of pageable requests come from a stream-ordered pool
+        // of the server's own, which keeps what is freed cached instead of
+        // returning it at every synchronisation (the device's default pool,
+        // shared with the rest of the process, is left as it is)
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = sv->device;
+        rc = hip_rc(hipMemPoolCreate(&sv->pool, &props));
+        if (rc == RMQ_OK) {
             uint64_t keep = UINT64_MAX;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            rc = hip_rc(hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep));
         }
     }
     size_t free_b = 0, total_b = 0;
@@ -697,14 +705,9 @@ RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t 
     if (dev != sv->device) return RMQ_ERR_DEVICE;
     std::lock_guard<std::mutex> lk(sv->mu);
     if ((int32_t)sv->arrays.size() >= sv->descs_cap) return RMQ_ERR_OOM;
-    // admission of a new resident array: its block table must fit in free HBM
-    // beside the request budget still available
-    const int64_t nb = (n + bs - 1) / bs;
-    int64_t levels = 0;
-    while (((int64_t)1 << levels) <= nb) ++levels;
-    int64_t table = (int64_t)sizeof(Pair) * nb * levels;
-    if (flags & RMQ_BUILD_EDGES) table += (int64_t)sizeof(Pair) * (4 * n + n / 2);  // edge tables, generous
-    else if (bs > kSub) table += (int64_t)sizeof(Pair) * nb * ((bs + kSub - 1) / kSub);  // sub-block minima
+    // admission of a new resident array: its tables (exactly what rmq_build
+    // allocates) must fit in free HBM beside the request budget still available
+    const int64_t table = handle_sizes(n, bs, flags).total();
     size_t free_b = 0, total_b = 0;
     int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
     if (rc != RMQ_OK) return rc;

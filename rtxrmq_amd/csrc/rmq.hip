@@ -564,16 +564,6 @@ using namespace rmqx;
 
 namespace {
 
-// Edge tables in one allocation: pre_b, suf_b (8 B per element each), and when
-// bs > 32 also pre_s, suf_s (the same inside sub-blocks), the sub-block sparse
-// table and the two per-sub-block exclusive scans.
-// floor(log2(nsub)) + 1 rows of the sparse table over sub-block minima
-int sub_levels(int nsub) {
-    int levels_s = 0;
-    while ((1 << levels_s) <= nsub) ++levels_s;
-    return levels_s;
-}
-
 // Rows 1 .. levels_s-1 of the sparse table over the subs sub-block minima in
 // row 0 (one flat sequence: windows may cross block boundaries).
 int build_sub_levels(Pair *st_s, int64_t subs, int levels_s, hipStream_t s) {
@@ -588,15 +578,19 @@ int build_sub_levels(Pair *st_s, int64_t subs, int levels_s, hipStream_t s) {
     return rc;
 }
 
-int build_edges(rmq_handle *h, hipStream_t s) {
+// Edge tables in one allocation (handle_sizes): pre_b, suf_b (8 B per
+// element each), and when bs > 32 also pre_s, suf_s (the same inside
+// sub-blocks), the sub-block sparse table and the two per-sub-block
+// exclusive scans.
+int build_edges(rmq_handle *h, const HandleSizes &z, hipStream_t s) {
     const int64_t n = h->n;
     const int bs = h->bs;
-    const int nsub = (bs + kSub - 1) / kSub;
-    const int64_t subs = h->nb * nsub;
-    const int levels_s = sub_levels(nsub);
+    const int nsub = z.nsub;
+    const int64_t subs = z.subs;
+    const int levels_s = z.levels_s;
     const bool two_level = bs > kSub;
     const int64_t per_elem = two_level ? 4 : 2;
-    const int64_t bytes = (int64_t)sizeof(Pair) * (per_elem * n + (int64_t)levels_s * subs + 2 * subs);
+    const int64_t bytes = z.edge_bytes;
     int rc = hip_rc(hipMalloc(&h->edge_mem, (size_t)bytes));
     if (rc != RMQ_OK) return rc;
     h->edge_bytes = bytes;
@@ -633,9 +627,9 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES) ||
         (reinterpret_cast<uintptr_t>(d_A) & 15))
         return RMQ_ERR_INVALID_ARG;
-    const int64_t nb = (n + bs - 1) / bs;
-    int32_t levels = 0;
-    while (((int64_t)1 << levels) <= nb) ++levels;  // floor(log2(nb)) + 1
+    const HandleSizes z = handle_sizes(n, bs, flags);
+    const int64_t nb = z.nb;
+    const int32_t levels = z.levels;
 
     rmq_handle *h = new (std::nothrow) rmq_handle{};
     if (!h) return RMQ_ERR_OOM;
@@ -644,7 +638,7 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     h->bs = bs;
     h->nb = nb;
     h->levels = levels;
-    h->st_bytes = (int64_t)sizeof(Pair) * nb * levels;
+    h->st_bytes = z.st_bytes;
     int rc = hip_rc(hipGetDevice(&h->device));
     if (rc == RMQ_OK) rc = hip_rc(hipMalloc(&h->st, (size_t)h->st_bytes));
     if (rc == RMQ_OK) {
@@ -667,19 +661,19 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
         rc = hip_rc(hipGetLastError());
     }
     h->flags = flags;
-    if (rc == RMQ_OK && (flags & RMQ_BUILD_EDGES)) rc = build_edges(h, s);
+    if (rc == RMQ_OK && (flags & RMQ_BUILD_EDGES)) rc = build_edges(h, z, s);
     // the sparse table over sub-block minima (32 elements, restarting at
     // every block start; row 0 the minima) for long scans: the edge tables'
     // when built, else an allocation of its own, 8 bytes per 32 elements and
     // level
     if (rc == RMQ_OK && bs > kSub) {
-        h->nsub = (bs + kSub - 1) / kSub;
+        h->nsub = z.nsub;
         if (h->edges.st_s) {
             h->sub = const_cast<Pair *>(h->edges.st_s);
         } else {
-            const int64_t subs = nb * h->nsub;
-            const int levels_s = sub_levels(h->nsub);
-            h->sub_bytes = (int64_t)sizeof(Pair) * subs * levels_s;
+            const int64_t subs = z.subs;
+            const int levels_s = z.levels_s;
+            h->sub_bytes = z.sub_bytes;
             rc = hip_rc(hipMalloc(&h->sub_mem, (size_t)h->sub_bytes));
             if (rc == RMQ_OK) {
                 h->sub = static_cast<Pair *>(h->sub_mem);
@@ -753,6 +747,12 @@ RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int6
     if (nb) *nb = h->nb;
     if (levels) *levels = h->levels;
     if (bytes) *bytes = h->st_bytes + h->edge_bytes + h->sub_bytes;
+    return RMQ_OK;
+}
+
+RMQ_API int rmq_handle_bytes(int64_t n, int32_t bs, uint32_t flags, int64_t *bytes) {
+    if (!bytes || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES)) return RMQ_ERR_INVALID_ARG;
+    *bytes = handle_sizes(n, bs, flags).total();
     return RMQ_OK;
 }
 
@@ -931,6 +931,44 @@ RMQ_API int rmq_gen_queries(int32_t *d_Q, int64_t q, int32_t n, int32_t lr, uint
 // ===========================================================================
 namespace {
 
+// Cache-key version: bump whenever the tuner's search or timing changes, so
+// picks made by another tuner never match.
+constexpr const char *kTuneVersion = "t2";
+constexpr int kSigWindow = 64;     // elements per sampled window of the array
+constexpr int kSigWindows = 4096;  // windows sampled, evenly spaced
+
+// Value signature of an array: over kSigWindows evenly spaced windows of
+// kSigWindow elements, the mean position of the leftmost minimum within its
+// window (0 ascending, 0.5 i.i.d., 1 descending; in quarters) and
+// floor(log2(1 + mean count of further elements equal to the window
+// minimum)) (0 for distinct values, 2 for 16 distinct values).  Both move
+// the superset test's success rate, so arrays that differ in them are tuned
+// apart.
+int value_signature(const float *d_A, int64_t n, hipStream_t s, char *out, size_t out_len) {
+    const int w = (int)std::min<int64_t>(n, kSigWindow);
+    const int64_t m = std::min<int64_t>(kSigWindows, n / w);
+    const int64_t step = (n - w) / std::max<int64_t>(1, m - 1);
+    std::vector<float> h((size_t)(m * w));
+    // window k starts at k * step (step >= w), the last one ends at or before n
+    RMQ_HIP(hipMemcpy2DAsync(h.data(), sizeof(float) * (size_t)w, d_A, sizeof(float) * (size_t)std::max<int64_t>(step, w),
+                             sizeof(float) * (size_t)w, (size_t)m, hipMemcpyDeviceToHost, s));
+    RMQ_HIP(hipStreamSynchronize(s));
+    double pos = 0.0, extra = 0.0;
+    for (int64_t k = 0; k < m; ++k) {
+        const float *x = h.data() + k * w;
+        int am = 0, eq = 0;
+        for (int j = 1; j < w; ++j)
+            if (x[j] < x[am]) am = j;
+        for (int j = 0; j < w; ++j) eq += x[j] == x[am];
+        pos += w > 1 ? (double)am / (w - 1) : 0.5;
+        extra += eq - 1;
+    }
+    pos /= (double)m;
+    extra /= (double)m;
+    snprintf(out, out_len, "pos%.2f/tie%d", std::round(4.0 * pos) / 4.0, (int)std::floor(std::log2(1.0 + extra)));
+    return RMQ_OK;
+}
+
 // Length-distribution signature of a query batch: log2 n and the 10th, 50th
 // and 90th percentiles of log2(range length), in half steps, over up to 2^16
 // evenly spaced queries.
@@ -1027,9 +1065,14 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
         (n_bs > 0 && !bs_list))
         return RMQ_ERR_INVALID_ARG;
     *out = rmq_tune_result_t{};
-    char key[160];
-    int rc = tune_signature(d_Q, q, n, s, key, sizeof(key));
+    // cache key: tuner version / n and length percentiles / value signature /
+    // outputs timed (value+index, value or index)
+    char lens[96], vals[32], key[160];
+    int rc = tune_signature(d_Q, q, n, s, lens, sizeof(lens));
+    if (rc == RMQ_OK) rc = value_signature(d_A, n, s, vals, sizeof(vals));
     if (rc != RMQ_OK) return rc;
+    snprintf(key, sizeof(key), "%s/%s/%s/out=%s", kTuneVersion, lens, vals, d_val && d_idx ? "vi" : (d_val ? "v" : "i"));
+    if (strlen(key) >= sizeof(out->key)) return RMQ_ERR_INVALID_ARG;
     if (cache_lookup(cache_path, key, out)) {
         out->from_cache = 1;
         out->n_tried = 0;

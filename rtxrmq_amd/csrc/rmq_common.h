@@ -617,6 +617,46 @@ inline int device_cus() {
     return cus;
 }
 
+// floor(log2(count)) + 1: rows of a sparse table over count entries
+inline int table_levels(int64_t count) {
+    int levels = 0;
+    while (((int64_t)1 << levels) <= count) ++levels;
+    return levels;
+}
+
+// Device bytes an index over n elements in blocks of bs owns, per allocation
+// (rmq_build allocates exactly these; the server's admission check uses
+// the same figures):
+//   st   -- the block sparse table, levels * nb pairs;
+//   sub  -- without edge tables and bs > 32, the sparse table over the
+//           32-element sub-block minima, levels_s * nb * nsub pairs;
+//   edge -- with edge tables: 2 (bs <= 32) or 4 per-element pair arrays,
+//           the sub-block sparse table (levels_s rows of nb * nsub pairs)
+//           and two per-sub-block exclusive scans used by the build.
+struct HandleSizes {
+    int64_t nb, subs;
+    int32_t levels, nsub, levels_s;
+    int64_t st_bytes, sub_bytes, edge_bytes;
+    int64_t total() const { return st_bytes + sub_bytes + edge_bytes; }
+};
+
+inline HandleSizes handle_sizes(int64_t n, int32_t bs, uint32_t flags) {
+    HandleSizes z{};
+    z.nb = (n + bs - 1) / bs;
+    z.levels = table_levels(z.nb);
+    z.nsub = (bs + kSub - 1) / kSub;
+    z.subs = z.nb * z.nsub;
+    z.levels_s = table_levels(z.nsub);
+    z.st_bytes = (int64_t)sizeof(Pair) * z.nb * z.levels;
+    if (flags & RMQ_BUILD_EDGES) {
+        const int64_t per_elem = bs > kSub ? 4 : 2;
+        z.edge_bytes = (int64_t)sizeof(Pair) * (per_elem * n + (int64_t)z.levels_s * z.subs + 2 * z.subs);
+    } else if (bs > kSub) {
+        z.sub_bytes = (int64_t)sizeof(Pair) * z.subs * z.levels_s;
+    }
+    return z;
+}
+
 inline int hip_rc(hipError_t e) {
     if (e == hipSuccess) return RMQ_OK;
     if (e == hipErrorOutOfMemory) return RMQ_ERR_OOM;

@@ -14,8 +14,11 @@ in librtxrmq.so.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
 import time
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -70,6 +73,40 @@ def _dev_out(t, dtype: str, q: int, device, name: str):
     return t
 
 
+# Native objects still open.  They are closed by an exit hook registered at
+# the first one's creation (so it runs before the exit hooks of modules
+# imported earlier, torch's included): servers first (their threads, streams
+# and pinned buffers), then plans, then indexes, all while the interpreter
+# and the HIP runtime are still whole.  During interpreter finalization the
+# finalizers do nothing: module globals (ctypes, this module's library
+# handle) may already be gone by then, and the HIP runtime's own teardown
+# follows.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+_EXIT_HOOK = [False]
+
+
+def _track(obj) -> None:
+    if not _EXIT_HOOK[0]:
+        atexit.register(_close_all)
+        _EXIT_HOOK[0] = True
+    _LIVE.add(obj)
+
+
+def _close_all() -> None:
+    objs = list(_LIVE)
+    for kind in (Server, Plan, RMQIndex):
+        for o in objs:
+            if isinstance(o, kind):
+                try:
+                    o.close()
+                except Exception:  # noqa: BLE001 -- best effort at exit
+                    pass
+
+
+def _finalizing() -> bool:
+    return sys.is_finalizing()
+
+
 def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0, long_lanes: int = 0) -> N.QueryOpts:
     return N.QueryOpts(short_max, group_lanes, long_wgs, long_lanes)
 
@@ -89,6 +126,7 @@ class RMQIndex:
         N.check(N.lib().rmq_build(A.data_ptr(), self.n, self.bs, self.flags, _stream_ptr(stream), ctypes.byref(h)),
                 "rmq_build")
         self._h = h
+        _track(self)
 
     @property
     def handle(self) -> int:
@@ -154,9 +192,11 @@ class RMQIndex:
             self._h = ctypes.c_void_p()
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.close()
-        except Exception:
+        except Exception:  # noqa: BLE001
             pass
 
     def __enter__(self):
@@ -164,6 +204,13 @@ class RMQIndex:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def handle_bytes(n: int, bs: int, flags: int = 0) -> int:
+    """Device bytes rmq_build(n, bs, flags) allocates (rmq_handle_bytes; no GPU needed)."""
+    b = ctypes.c_int64()
+    N.check(N.lib().rmq_handle_bytes(n, bs, flags, ctypes.byref(b)), "rmq_handle_bytes")
+    return b.value
 
 
 def gpu_base(A, Q, want_value: bool = True, want_index: bool = True, stream=None):
@@ -189,24 +236,31 @@ def check_queries(Q, n: int, stream=None) -> int:
     return bad.value
 
 
-def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool = False, stream=None) -> dict:
-    """Autotuner pick (rmq_tune) for this array and a sample of its queries.
+def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool = False, stream=None,
+         want_value: bool = True, want_index: bool = True) -> dict:
+    """Autotuner pick (rmq_tune) for this array and a sample of its queries,
+    timing the outputs the caller will ask for (value and/or index).
 
-    With a cache file, a pick stored for the same (n, length signature) is
-    returned without timing anything; refresh=True times the sweep anyway and
-    appends the new pick (the last line for a key wins).
+    With a cache file, a pick stored for the same key (tuner version, n,
+    length percentiles, value signature, outputs) is returned without timing
+    anything; refresh=True times the sweep anyway and appends the new pick
+    (the last line for a key wins).
     Returns {"bs", "flags" (rmq_build flags), "opts" (QueryOpts), "best_ms", "n_tried", "from_cache", "key"}.
     """
     torch = _torch()
     A = _dev_f32(A, "A")
     Q_sample = _dev_queries(Q_sample)
+    if not (want_value or want_index):
+        raise ValueError("tune: time at least one output")
     q = Q_sample.numel() // 2
-    scratch = torch.empty(q, dtype=torch.int32, device=A.device)
+    v_scr = torch.empty(q, dtype=torch.float32, device=A.device) if want_value else None
+    i_scr = torch.empty(q, dtype=torch.int32, device=A.device) if want_index else None
     arr = (ctypes.c_int32 * len(bs_list))(*bs_list) if bs_list else None
     res = N.TuneResult()
     use_cache = cache_path if (cache_path and not refresh) else None
     rc = N.lib().rmq_tune(A.data_ptr(), A.numel(), Q_sample.data_ptr(), q, arr, len(bs_list) if bs_list else 0,
-                          None, scratch.data_ptr(), use_cache.encode() if use_cache else None,
+                          v_scr.data_ptr() if v_scr is not None else None,
+                          i_scr.data_ptr() if i_scr is not None else None, use_cache.encode() if use_cache else None,
                           _stream_ptr(stream), ctypes.byref(res))
     N.check(rc, "rmq_tune")
     o = res.opts
@@ -299,6 +353,7 @@ class Plan:
                                      item_queries, _stream_ptr(stream), ctypes.byref(h))
         N.check(rc, "rmq_plan_create")
         self._h = h
+        _track(self)
 
     def launch(self, stream=None):
         N.check(N.lib().rmq_plan_launch(self._h, _stream_ptr(stream)), "rmq_plan_launch")
@@ -317,9 +372,11 @@ class Plan:
             self._h = ctypes.c_void_p()
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.close()
-        except Exception:
+        except Exception:  # noqa: BLE001
             pass
 
     def __enter__(self):
@@ -346,6 +403,7 @@ class Server:
         h = ctypes.c_void_p()
         N.check(N.lib().rmq_server_create(ctypes.byref(cfg), ctypes.byref(h)), "rmq_server_create")
         self._h = h
+        _track(self)
         self._arrays = []
         self._live = {}
 
@@ -403,9 +461,11 @@ class Server:
             self._live.clear()
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.close()
-        except Exception:
+        except Exception:  # noqa: BLE001
             pass
 
     def __enter__(self):

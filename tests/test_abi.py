@@ -26,6 +26,25 @@ def test_library_loads_and_exports_header():
     assert set(names) <= exported
 
 
+def test_handle_bytes_without_a_gpu():
+    """rmq_handle_bytes is host arithmetic: validated arguments, and for
+    bs <= 32 without edge tables only the block sparse table (nb pairs times
+    floor(log2 nb) + 1 levels, 8 bytes each)."""
+    lib = R.lib()
+    b = ctypes.c_int64()
+    assert lib.rmq_handle_bytes(0, 4, 0, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(10, 0, 0, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(10, 4, 7, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(1 << 31, 4, 0, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(10, 4, 0, None) == R.RMQ_ERR_INVALID_ARG
+    for n, bs in ((1, 1), (100, 8), (1000, 32), (12345, 7)):
+        nb = -(-n // bs)
+        assert R.handle_bytes(n, bs) == 8 * nb * nb.bit_length()
+    # edge tables add per-element pairs: 2 per element up to bs = 32, 4 above
+    assert R.handle_bytes(1 << 20, 32, R.BUILD_EDGES) - R.handle_bytes(1 << 20, 32) >= 16 << 20
+    assert R.handle_bytes(1 << 20, 1024, R.BUILD_EDGES) - R.handle_bytes(1 << 20, 1024, 0) >= 24 << 20
+
+
 def test_strerror_and_version():
     assert R.strerror(R.RMQ_OK) == "ok"
     for code in (R.RMQ_ERR_INVALID_ARG, R.RMQ_ERR_OOM, R.RMQ_ERR_HIP, R.RMQ_ERR_DEVICE, R.RMQ_ERR_BAD_QUERY):

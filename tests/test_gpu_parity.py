@@ -157,11 +157,44 @@ def test_autotuner_pick_and_cache(tmp_path, gpu):
     assert r1["opts"].group_lanes in (4, 8, 16, 64) and r1["opts"].long_lanes in (4, 8, 16, 64)
     r2 = R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)
     assert r2["from_cache"] and r2["bs"] == r1["bs"] and r2["key"] == r1["key"]
+    assert r1["key"].startswith("t2/n2^22.0/") and r1["key"].endswith("/pos0.50/tie0/out=vi"), r1["key"]
     with R.RMQIndex(A_dev, r1["bs"]) as ix:
         v, i = ix.query(Q_dev, opts=r1["opts"])
         torch.cuda.synchronize()
     ev, ei = O.rmq_brute(A_dev.cpu().numpy(), Q)
     assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, "tuned config")
+
+
+def test_tuner_key_separates_value_kinds_and_outputs(tmp_path, gpu):
+    """Arrays that differ in tie density or trend, and tunings that time
+    different outputs, get different cache keys: a pick made for one never
+    serves the other."""
+    n = 1 << 20
+    Q_dev = _dev(R.gen_queries(1 << 14, n, -2, 9), gpu)
+    cache = str(tmp_path / "tune.txt")
+    keys = {}
+    for kind in ("uniform", "ties16", "desc"):
+        A_dev = R.gen_array(n, 9, kind, device=gpu)
+        keys[kind] = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache)["key"]
+    assert keys["uniform"].endswith("/pos0.50/tie0/out=vi"), keys
+    assert "/tie2/" in keys["ties16"] and "/pos0.25/" in keys["ties16"], keys
+    assert keys["desc"].endswith("/pos1.00/tie0/out=vi"), keys
+    A_dev = R.gen_array(n, 9, "uniform", device=gpu)
+    r_i = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache, want_value=False)
+    r_v = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache, want_index=False)
+    assert r_i["key"].endswith("/out=i") and r_v["key"].endswith("/out=v")
+    assert not r_i["from_cache"] and not r_v["from_cache"]
+
+
+@pytest.mark.parametrize("flags", FLAGS)
+@pytest.mark.parametrize("bs", [33, 1024, 1 << 15, 1 << 18, 1 << 20])
+def test_handle_bytes_is_what_the_build_allocates(bs, flags, gpu):
+    """rmq_handle_bytes (the server's admission figure) equals the bytes the
+    built handle reports, for plain and edge-table handles."""
+    n = 3_000_017
+    A_dev = R.gen_array(n, 3, device=gpu)
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
+        assert R.handle_bytes(n, bs, flags) == ix.info()["bytes"]
 
 
 @pytest.mark.parametrize("n", [1, 5, 7])
