@@ -43,6 +43,7 @@ typedef struct rmq_handle rmq_handle_t;
 #define RMQ_ERR_HIP (-3)
 #define RMQ_ERR_DEVICE (-4)    /* handle used on a device other than the one it was built on */
 #define RMQ_ERR_BAD_QUERY (-5) /* returned by rmq_query_check when a query had l > r or r >= n */
+#define RMQ_ERR_LIMIT (-6)     /* beyond the static scratch of rmq_plan_launch_static */
 
 /* Array kinds for the generators (the reference draws U(0,1] with cuRAND,
  * src/rand.cuh:15-21; these draw exact grid values from a counter-based
@@ -192,6 +193,26 @@ int rmq_plan_launch(const rmq_plan_t *p, hipStream_t s);
 int rmq_plan_graph_launch(rmq_plan_t *p, hipStream_t s);
 int rmq_plan_info(const rmq_plan_t *p, int32_t *n_arrays, int32_t *n_items, int64_t *n_queries);
 void rmq_plan_free(rmq_plan_t *p);
+/* ---- allocation-free form of rmq_plan_create + rmq_plan_launch (no
+ *      counterpart in the reference): for callers that cannot allocate
+ *      device memory (tight serving loops, code that must not call hipMalloc).
+ *      Same arguments and answers as a plan over the same handles and
+ *      buffers, launched on s; nothing is allocated or freed, on the device
+ *      or the host.  The plan's device state (queue head, one descriptor per
+ *      handle, the work list) lives in a static device array of the library,
+ *      about 2.9 MB, which each GPU holds until the process ends: at most
+ *      8192 arrays and 65536 work items (item_queries 0 sizes items as the
+ *      plan does and enlarges them to stay within the limit; an explicit
+ *      item_queries that passes it, or more arrays, return RMQ_ERR_LIMIT).
+ *      The work list is uploaded from a static host image and the call waits
+ *      for that upload on s (work already queued on s runs first); the query
+ *      kernel itself is asynchronous.  Every call on one GPU shares that
+ *      GPU's array: calls on the same stream are ordered, calls that overlap
+ *      on different streams of one GPU race.  Not capturable in a HIP graph
+ *      (it synchronises s). ---- */
+int rmq_plan_launch_static(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs,
+                           const int64_t *qs, float *const *d_vals, int32_t *const *d_idxs,
+                           const rmq_query_opts_t *opts, int32_t item_queries, hipStream_t s);
 /* The per-array launch loop the plan is measured against: rmq_query_ex on
  * each array in turn, same arguments as rmq_plan_create. */
 int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs, const int64_t *qs,

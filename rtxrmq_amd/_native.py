@@ -21,6 +21,7 @@ RMQ_ERR_OOM = -2
 RMQ_ERR_HIP = -3
 RMQ_ERR_DEVICE = -4
 RMQ_ERR_BAD_QUERY = -5
+RMQ_ERR_LIMIT = -6
 
 GEN_KINDS = {"uniform": 0, "ties16": 1, "desc": 2, "const": 3}
 BUILD_EDGES = 1  # RMQ_BUILD_EDGES
@@ -104,6 +105,8 @@ def _declare(lib):
         "rmq_plan_info": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
         "rmq_plan_free": (None, [_vp]),
         "rmq_query_loop": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _vp]),
+        "rmq_plan_launch_static": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(QueryOpts), _i32,
+                                                  _vp]),
         "rmq_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
         "rmq_server_add_array": (ctypes.c_int, [_vp, _vp, _i64, _i32, ctypes.POINTER(_i32)]),
         "rmq_server_add_array_ex": (ctypes.c_int, [_vp, _vp, _i64, _i32, _u32, ctypes.POINTER(_i32)]),

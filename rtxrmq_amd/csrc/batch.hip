@@ -295,6 +295,123 @@ RMQ_API int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const 
 }
 
 // ===========================================================================
+// allocation-free batched launch: the plan's device state (queue head, array
+// descriptors, work items) in a static device array instead of allocations
+// ===========================================================================
+#include <mutex>
+
+namespace rmqx {
+constexpr int32_t kStaticMaxArrays = 8192;
+constexpr int32_t kStaticMaxItems = 1 << 16;
+constexpr size_t kStaticDescOff = 256;  // bytes [0, 256): the queue head
+constexpr size_t kStaticItemOff = kStaticDescOff + (sizeof(ArrayDesc) * kStaticMaxArrays + 255) / 256 * 256;
+constexpr size_t kStaticBytes = kStaticItemOff + sizeof(WorkItem) * kStaticMaxItems;  // about 2.9 MB
+}  // namespace rmqx
+
+// The device state of rmq_plan_launch_static.  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code this name is
+// not a device address: its address on a device comes from
+// hipGetSymbolAddress (static_scratch below).
+static __device__ __align__(256) unsigned char scratch[rmqx::kStaticBytes];
+
+namespace {
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_static_mu;                                  // one host staging buffer, one caller at a time
+unsigned char *g_static_dev[kMaxStaticDevices];          // device address of `scratch`, per device
+alignas(256) unsigned char g_static_stage[rmqx::kStaticBytes];  // host image of the descriptors and items
+
+// This device's address of `scratch`, looked up on the device's first call
+// (with that device current) and kept.  Caller holds g_static_mu.
+int static_scratch(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= kMaxStaticDevices) return RMQ_ERR_DEVICE;
+    if (!g_static_dev[dev]) {
+        void *p = nullptr;
+        const int rc = hip_rc(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (rc != RMQ_OK) return rc;
+        if (!p) return RMQ_ERR_HIP;
+        g_static_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_static_dev[dev];
+    return RMQ_OK;
+}
+
+}  // namespace
+
+RMQ_API int rmq_plan_launch_static(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs,
+                                   const int64_t *qs, float *const *d_vals, int32_t *const *d_idxs,
+                                   const rmq_query_opts_t *opts, int32_t item_queries, hipStream_t s) {
+    if (m > kStaticMaxArrays) return RMQ_ERR_LIMIT;
+    if (m < 0 || (m > 0 && (!handles || !d_Qs || !qs)) || item_queries < 0) return RMQ_ERR_INVALID_ARG;
+    int dev = -1;
+    RMQ_HIP(hipGetDevice(&dev));
+    rmq_query_opts_t o;
+    rmq_handle dummy{};
+    int rc = resolve_opts(m > 0 ? handles[0] : &dummy, opts, &o);
+    if (rc != RMQ_OK) return rc;
+    int64_t total = 0;
+    for (int32_t i = 0; i < m; ++i) {
+        const rmq_handle *h = handles[i];
+        const bool has_out = (d_vals && d_vals[i]) || (d_idxs && d_idxs[i]);
+        if (!h || qs[i] < 0 || qs[i] > INT32_MAX || (qs[i] > 0 && (!d_Qs[i] || !has_out))) return RMQ_ERR_INVALID_ARG;
+        if (h->device != dev) return RMQ_ERR_DEVICE;
+        total += qs[i];
+    }
+    if (total == 0) return RMQ_OK;
+    // work items: as rmq_plan_create sizes them, made larger when that would
+    // pass the static limit (an explicit size that passes it is refused)
+    int64_t iq = item_queries > 0 ? item_queries : auto_item_queries(total);
+    auto count_items = [&](int64_t per) {
+        int64_t c = 0;
+        for (int32_t i = 0; i < m; ++i) c += (qs[i] + per - 1) / per;
+        return c;
+    };
+    int64_t n_items = count_items(iq);
+    if (n_items > kStaticMaxItems) {
+        if (item_queries > 0) return RMQ_ERR_LIMIT;
+        while (n_items > kStaticMaxItems) n_items = count_items(iq *= 2);
+    }
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    unsigned char *d_scr = nullptr;
+    if ((rc = static_scratch(dev, &d_scr)) != RMQ_OK) return rc;
+    ArrayDesc *h_descs = reinterpret_cast<ArrayDesc *>(g_static_stage + kStaticDescOff);
+    WorkItem *h_items = reinterpret_cast<WorkItem *>(g_static_stage + kStaticItemOff);
+    size_t k = 0;
+    for (int32_t i = 0; i < m; ++i) {
+        h_descs[i] = desc_of(handles[i]);
+        for (int64_t off = 0; off < qs[i]; off += iq)
+            h_items[k++] = WorkItem{reinterpret_cast<const int2 *>(d_Qs[i]) + off,
+                                    (d_vals && d_vals[i]) ? d_vals[i] + off : nullptr,
+                                    (d_idxs && d_idxs[i]) ? d_idxs[i] + off : nullptr,
+                                    (int32_t)std::min<int64_t>(iq, qs[i] - off), i};
+    }
+    // longest items first, as in rmq_plan_create (in place: no allocation)
+    auto cost = [&](const WorkItem &w) { return (double)w.count * std::log2(2.0 + (double)handles[w.array]->n); };
+    std::sort(h_items, h_items + k, [&](const WorkItem &a, const WorkItem &b) {
+        const double ca = cost(a), cb = cost(b);
+        if (ca != cb) return ca > cb;
+        return a.array != b.array ? a.array < b.array : a.Q < b.Q;
+    });
+    // upload, then wait for it: the host image is reused by the next call
+    rc = hip_rc(hipMemcpyAsync(d_scr + kStaticDescOff, h_descs, sizeof(ArrayDesc) * (size_t)m, hipMemcpyHostToDevice, s));
+    if (rc == RMQ_OK)
+        rc = hip_rc(hipMemcpyAsync(d_scr + kStaticItemOff, h_items, sizeof(WorkItem) * k, hipMemcpyHostToDevice, s));
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    if (rc != RMQ_OK) return rc;
+    unsigned *d_head = reinterpret_cast<unsigned *>(d_scr);
+    k_reset_head<<<1, 64, 0, s>>>(d_head);  // a kernel, not a memset
+    rc = hip_rc(hipGetLastError());
+    if (rc != RMQ_OK) return rc;
+    int grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)k + kWavesPerWG - 1) / kWavesPerWG,
+                                                           (int64_t)device_cus() * 8));
+    if (o.long_wgs > 0) grid = std::min(grid, o.long_wgs);
+    hipLaunchKernelGGL(pick_kernel(o.group_lanes, o.long_lanes), dim3(grid), dim3(kWG), 0, s,
+                       reinterpret_cast<const ArrayDesc *>(d_scr + kStaticDescOff),
+                       reinterpret_cast<const WorkItem *>(d_scr + kStaticItemOff), (int)k, d_head, o.short_max);
+    return hip_rc(hipGetLastError());
+}
+
+// ===========================================================================
 // serving loop: continuous batching over resident arrays
 // ===========================================================================
 #include <chrono>

@@ -1331,6 +1331,9 @@ RMQ_API const char *rmq_strerror(int code) {
     case RMQ_ERR_HIP: return "HIP runtime error";
     case RMQ_ERR_DEVICE: return "handle used on a device other than the one it was built on";
     case RMQ_ERR_BAD_QUERY: return "query with l < 0, l > r or r >= n";
+    case RMQ_ERR_LIMIT:
+        return "more arrays or work items than the static scratch of rmq_plan_launch_static holds (8192 arrays, "
+               "65536 work items)";
     default: return "unknown error";
     }
 }
