@@ -282,6 +282,26 @@ int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset);
  * wait on the server during or after this call. */
 void rmq_server_destroy(rmq_server_t *sv);
 
+/* ---- host HRMQ, algs 1/101: replaces rmq_rmm_par / rmq_rmm_par_idx
+ *      (src/cpu_methods.h:80-159), which build the authors' RMQRMM64
+ *      (hrmq/RMQRMM64.cpp:196-251) single-threaded and answer queryRMQ
+ *      (hrmq/RMQRMM64.cpp:1463-1465) in an OpenMP loop.  A restatement of
+ *      the same published structure (Ferrada & Navarro 2017: balanced
+ *      parentheses of the 2d-min-heap, rank/select, a range min-tree over
+ *      the excess).  A holds n values compared as int32 (the reference reads
+ *      the float array as int*, src/main.cu:117-125) and must outlive the
+ *      handle.  Answers: idx = the RIGHTMOST minimum of A[l..r] (HRMQ's tie
+ *      rule), val = A[idx] as int32 bits; invalid ranges give -1 and the
+ *      bits of NaN.  rmq_hrmq_query runs on nthreads host threads (0: all
+ *      cores) in static chunks, as the reference's loop; either output may
+ *      be NULL but not both.  No GPU is involved. ---- */
+typedef struct rmq_hrmq rmq_hrmq_t;
+int rmq_hrmq_build(const int32_t *A, int64_t n, rmq_hrmq_t **out);
+int rmq_hrmq_query(const rmq_hrmq_t *h, const int32_t *Q, int64_t q, int32_t *idx, int32_t *val, int32_t nthreads);
+/* Bytes the structure holds (the reference prints RMQRMM64::getSize). */
+int rmq_hrmq_bytes(const rmq_hrmq_t *h, int64_t *bytes);
+void rmq_hrmq_free(rmq_hrmq_t *h);
+
 /* ---- brute force on the device, algs 2/102 and the --check oracle:
  *      replaces kernel_rmq_basic / kernel_rmq_basic_idx / gpu_rmq_basic
  *      (src/cuda_methods.cuh:3-169): one thread per query, linear scan from

@@ -8,7 +8,7 @@ from ._native import (BUILD_EDGES, GEN_KINDS, RMQ_ERR_BAD_QUERY, RMQ_ERR_DEVICE,
                       strerror)
 from .engine import (ALG_CPU_BASE, ALG_CPU_BASE_IDX, ALG_CPU_HRMQ, ALG_CPU_HRMQ_IDX, ALG_GPU_BASE,
                      ALG_GPU_BASE_IDX, ALG_GPU_RTX_BLOCKS, ALG_GPU_RTX_BLOCKS_IDX, RMQIndex, RunResult,
-                     Plan, Server, check_queries, gen_array, handle_bytes, gen_array_host, gen_queries, gen_queries_device, gen_queries_ref, gpu_base,
+                     Plan, Server, HostHRMQ, check_queries, gen_array, handle_bytes, gen_array_host, gen_queries, gen_queries_device, gen_queries_ref, gpu_base,
                      probe_gather, query_opts, rtx_rmq, tune)
 
 __all__ = [
@@ -16,5 +16,5 @@ __all__ = [
     "RMQ_ERR_BAD_QUERY", "QueryOpts", "QueryTimes", "RMQError", "check", "header_functions", "lib", "strerror",
     "ALG_CPU_BASE", "ALG_CPU_BASE_IDX", "ALG_CPU_HRMQ", "ALG_CPU_HRMQ_IDX", "ALG_GPU_BASE", "ALG_GPU_BASE_IDX",
     "ALG_GPU_RTX_BLOCKS", "ALG_GPU_RTX_BLOCKS_IDX", "RMQIndex", "RunResult", "check_queries", "gen_array",
-    "gen_array_host", "handle_bytes", "gen_queries", "gen_queries_device", "gen_queries_ref", "gpu_base", "probe_gather", "query_opts", "rtx_rmq", "tune", "Plan", "Server",
+    "gen_array_host", "handle_bytes", "HostHRMQ", "gen_queries", "gen_queries_device", "gen_queries_ref", "gpu_base", "probe_gather", "query_opts", "rtx_rmq", "tune", "Plan", "Server",
 ]

@@ -91,6 +91,10 @@ def _declare(lib):
         "rmq_query_opts_default": (ctypes.c_int, [_vp, ctypes.POINTER(QueryOpts)]),
         "rmq_query_kernels": (ctypes.c_int, [_vp, ctypes.POINTER(QueryOpts), ctypes.c_char_p, _i64]),
         "rmq_gpu_base": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
+        "rmq_hrmq_build": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(_vp)]),
+        "rmq_hrmq_query": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _i32]),
+        "rmq_hrmq_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
+        "rmq_hrmq_free": (None, [_vp]),
         "rmq_tune": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, ctypes.c_char_p, _vp,
                                     ctypes.POINTER(TuneResult)]),
         "rmq_gen_array": (ctypes.c_int, [_vp, _i64, _u64, _i32, _vp]),

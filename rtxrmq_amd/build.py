@@ -49,11 +49,17 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+# Host-only C++ (the CPU HRMQ of algs 1/101): g++, no device code.
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-mpopcnt", "-Wall", f"-I{INCLUDE}"]
+
+
 def build_lib() -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in ("rmq.hip", "batch.hip") if os.path.exists(os.path.join(CSRC, f))]
+    host_obj = os.path.join(LIB_DIR, "hrmq.o")
+    _run(["g++", *HOST_FLAGS, "-c", "-o", host_obj, os.path.join(CSRC, "hrmq.cpp")])
     tmp = LIB + ".tmp"
-    _run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs, "-lpthread"])
+    _run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs, "-x", "none", host_obj, "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

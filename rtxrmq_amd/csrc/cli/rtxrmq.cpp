@@ -4,7 +4,8 @@
 // reference driver (src/main.cu:51-171, src/tools.h:28-189,268-336), so its
 // scripts/*.sh sweeps and plotting scripts run unchanged.  The RTX-core algs
 // (3, 4, 6, 8, 9, 10, 103) are not part of this engine; alg 5 / 105 run the
-// HIP block engine, 2 / 102 the HIP brute force, 0 / 100 the host brute force.
+// HIP block engine, 2 / 102 the HIP brute force, 0 / 100 the host brute force,
+// 1 / 101 the host HRMQ (rmq_hrmq_*, rightmost minimum on ties).
 #include <getopt.h>
 #include <hip/hip_runtime.h>
 #include <unistd.h>
@@ -69,7 +70,7 @@ void usage() {
             "  lr   range length: >0 fixed; -1 uniform in [1,n-l]; -2 lognormal median 2^14;\n"
             "       -3 lognormal median 2^6; -4 / -5 lognormal n/2^8, n/2^15 (sigma 0.3); -6 r = n-1\n"
             "  alg  0 %s (values) | 100 same, indices\n"
-            "       1 %s | 101 indices  (host baseline; run by bench.py, not built in)\n"
+            "       1 %s | 101 indices  (host, --nt threads; rightmost index on ties)\n"
             "       2 %s | 102 indices\n"
             "       5 %s | 105 indices  (the HIP block engine)\n\n"
             "options:\n"
@@ -266,12 +267,8 @@ int main(int argc, char **argv) {
         return EXIT_FAILURE;
     }
     const bool idx_out = a.alg >= 100;
-    const bool cpu_alg = a.alg == ALG_CPU_BASE || a.alg == ALG_CPU_BASE_IDX;
-    if (a.alg == ALG_CPU_HRMQ || a.alg == ALG_CPU_HRMQ_IDX) {
-        fprintf(stderr, "alg %d (HRMQ) is the host baseline: it is timed by bench.py from the reference's own "
-                        "library, not built into this engine\n", a.alg);
-        return 2;
-    }
+    const bool hrmq_alg = a.alg == ALG_CPU_HRMQ || a.alg == ALG_CPU_HRMQ_IDX;
+    const bool cpu_alg = a.alg == ALG_CPU_BASE || a.alg == ALG_CPU_BASE_IDX || hrmq_alg;
     if (!cpu_alg && a.alg != ALG_GPU_BASE && a.alg != ALG_GPU_BASE_IDX && a.alg != ALG_GPU_RTX_BLOCKS &&
         a.alg != ALG_GPU_RTX_BLOCKS_IDX) {
         fprintf(stderr, "alg %d (%s) is an RT-core method outside this engine; use 5 / 105\n", a.alg,
@@ -361,14 +358,37 @@ int main(int argc, char **argv) {
     PowerSampler power;
     CpuEnergySampler cpu_power;
     if (cpu_alg) {
+        // HRMQ (src/cpu_methods.h:80-159): the structure is built on one
+        // thread and reported as construction; the values are the float
+        // array's bits read as int32 (src/main.cu:117-125)
+        rmq_hrmq_t *hr = nullptr;
+        if (hrmq_alg) {
+            printf("Creating MinMaxTree.......................");
+            fflush(stdout);
+            t0 = now_ms();
+            if (int rc = rmq_hrmq_build(reinterpret_cast<const int32_t *>(hA.data()), a.n, &hr)) die("rmq_hrmq_build", rc);
+            construction_ms = now_ms() - t0;
+            int64_t bytes = 0;
+            rmq_hrmq_bytes(hr, &bytes);
+            out_bytes = (double)bytes;
+            printf("done: %f ms (%f MB)\n", construction_ms, bytes / 1e6);
+        }
         printf(C_CYAN "Computing RMQs%s (%-16s,nt=%2i,r=%-3i).." C_RESET, idx_out ? " index" : "", alg_name(a.alg), a.nt,
                a.reps);
         fflush(stdout);
         if (a.save_power) cpu_power.start(a.power_file, alg_name(a.alg));
         t0 = now_ms();
-        for (int r = 0; r < a.reps; ++r)
-            cpu_base(a, hA.data(), hQ.data(), idx_out ? nullptr : out_v.data(), idx_out ? out_i.data() : nullptr);
+        for (int r = 0; r < a.reps; ++r) {
+            if (hrmq_alg) {
+                int rc = rmq_hrmq_query(hr, hQ.data(), a.q, idx_out ? out_i.data() : nullptr,
+                                        idx_out ? nullptr : reinterpret_cast<int32_t *>(out_v.data()), a.nt);
+                if (rc) die("rmq_hrmq_query", rc);
+            } else {
+                cpu_base(a, hA.data(), hQ.data(), idx_out ? nullptr : out_v.data(), idx_out ? out_i.data() : nullptr);
+            }
+        }
         time_ms = now_ms() - t0;
+        rmq_hrmq_free(hr);
         if (a.save_power) {
             cpu_power.stop();
             if (cpu_power.energy_j() > 0)
@@ -438,8 +458,14 @@ int main(int argc, char **argv) {
         hip_ok(hipMemcpy(ei.data(), d_ei, sizeof(int32_t) * (size_t)a.q, hipMemcpyDeviceToHost), "D2H");
         printf(C_YELLOW "Checking result..........................." C_RESET);
         int64_t bad = 0;
+        // HRMQ's index is the rightmost minimum (alg 101): it must hold the
+        // same value as the leftmost one GPU BASE returns and lie in [l, r]
         for (int64_t j = 0; j < a.q; ++j) {
-            const bool ok = idx_out ? out_i[j] == ei[j] : memcmp(&out_v[j], &ev[j], 4) == 0;
+            const int32_t l = hQ[2 * j], r = hQ[2 * j + 1];
+            const bool ok = !idx_out ? memcmp(&out_v[j], &ev[j], 4) == 0
+                            : a.alg == ALG_CPU_HRMQ_IDX
+                                ? out_i[j] >= l && out_i[j] <= r && memcmp(&hA[out_i[j]], &ev[j], 4) == 0
+                                : out_i[j] == ei[j];
             if (!ok && bad++ < 10) {
                 if (idx_out) printf("\nError on %lld-th query [%d,%d]: got %d, expected %d", (long long)j, hQ[2 * j],
                                     hQ[2 * j + 1], out_i[j], ei[j]);

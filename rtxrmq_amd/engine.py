@@ -94,7 +94,7 @@ def _track(obj) -> None:
 
 def _close_all() -> None:
     objs = list(_LIVE)
-    for kind in (Server, Plan, RMQIndex):
+    for kind in (Server, Plan, RMQIndex, HostHRMQ):
         for o in objs:
             if isinstance(o, kind):
                 try:
@@ -189,6 +189,53 @@ class RMQIndex:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             N.lib().rmq_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        if _finalizing():
+            return
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class HostHRMQ:
+    """The host HRMQ of algs 1/101 (rmq_hrmq_*): rightmost minimum index and
+    its value bits, over values compared as int32 (a float array is read as
+    its bit patterns, src/main.cu:117-125).  Runs on the CPU only."""
+
+    def __init__(self, A: np.ndarray):
+        self.A = np.ascontiguousarray(A).view(np.int32)
+        h = ctypes.c_void_p()
+        N.check(N.lib().rmq_hrmq_build(self.A.ctypes.data, self.A.size, ctypes.byref(h)), "rmq_hrmq_build")
+        self._h = h
+        _track(self)
+
+    def query(self, Q: np.ndarray, nthreads: int = 0, want_value: bool = True, want_index: bool = True):
+        """(value as the input's dtype or None, rightmost index or None)."""
+        Q = np.ascontiguousarray(Q, dtype=np.int32).reshape(-1, 2)
+        q = Q.shape[0]
+        idx = np.empty(q, np.int32) if want_index else None
+        val = np.empty(q, np.int32) if want_value else None
+        N.check(N.lib().rmq_hrmq_query(self._h, Q.ctypes.data, q, idx.ctypes.data if idx is not None else None,
+                                       val.ctypes.data if val is not None else None, nthreads), "rmq_hrmq_query")
+        return (val.view(np.float32) if val is not None else None), idx
+
+    def bytes(self) -> int:
+        b = ctypes.c_int64()
+        N.check(N.lib().rmq_hrmq_bytes(self._h, ctypes.byref(b)), "rmq_hrmq_bytes")
+        return b.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().rmq_hrmq_free(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

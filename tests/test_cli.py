@@ -36,8 +36,6 @@ def test_usage_and_argument_errors():
     assert r.returncode == 1 and "lr=2000 > n=1000" in r.stderr
     r = run(1000, 10, -1, 3)                            # RT-core algs are outside this engine
     assert r.returncode == 2 and "outside this engine" in r.stderr
-    r = run(1000, 10, -1, 1)                            # HRMQ is the baseline harness's
-    assert r.returncode == 2 and "HRMQ" in r.stderr
     assert run(1000, 10, -1, 5, "--array", "nope").returncode == 1
 
 
@@ -52,6 +50,42 @@ def test_cpu_base_writes_reference_csv_row(tmp_path, alg):
     assert len(row) == 13 and row[0].startswith("CPU ") and row[1] == "[CPU] BASE"
     assert row[2:7] == ["2", "3000", "0", "2000", "-3"]  # bs zeroed for non-block algs (src/tools.h:163-173)
     assert float(row[8]) > 0 and float(row[9]) > 0
+
+
+@pytest.mark.parametrize("alg", [1, 101])
+def test_hrmq_algs_write_reference_csv_row(tmp_path, alg):
+    """Algs 1/101 (host HRMQ, src/cpu_methods.h:80-159): construction and the
+    structure's size in the row's construction and outbuffer columns."""
+    csv = tmp_path / "h.csv"
+    r = run(200_000, 50_000, -2, alg, "--nt", 4, "--reps", 2, "--seed", 9, f"--save-time={csv}")
+    assert r.returncode == 0, r.stderr
+    assert "Creating MinMaxTree" in r.stdout and "Benchmark Finished" in r.stdout
+    row = csv.read_text().strip().split(",")
+    assert len(row) == 13 and row[0].startswith("CPU ") and row[1] == "[CPU] HRMQ"
+    assert row[2:7] == ["2", "200000", "0", "50000", "-2"]
+    assert float(row[8]) > 0 and float(row[10]) > 0 and float(row[11]) > 0
+
+
+def test_perf_fulltest_hrmq_lines_parse(tmp_path):
+    """The reference sweep's HRMQ invocations (scripts/perf-fulltest.sh:33-42
+    through scripts/perf-benchmark.sh:47-51): ./rtxrmq n q lr 1 --nb 1 --reps R
+    --nt N --dev D --save-time=FILE --seed S, at small sizes."""
+    csv = tmp_path / "p.csv"
+    for lr in (-1, -2, -3):
+        r = run(1 << 16, 1 << 14, lr, 1, "--nb", 1, "--reps", 2, "--nt", 2, "--dev", 0, f"--save-time={csv}",
+                "--seed", 1321)
+        assert r.returncode == 0, r.stderr
+    assert len(csv.read_text().strip().splitlines()) == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", [1, 101])
+@pytest.mark.parametrize("kind", ["uniform", "ties16"])
+def test_hrmq_algs_pass_check(alg, kind, gpu):
+    """--check against GPU BASE: values bit for bit (alg 1); for alg 101 the
+    rightmost index must hold the minimum value inside [l, r]."""
+    r = run(1 << 20, 1 << 16, -2, alg, "--nt", 8, "--reps", 1, "--seed", 3, "--array", kind, "--check")
+    assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
 
 
 def test_reference_query_stream_option(tmp_path):

@@ -259,6 +259,10 @@ def test_headline_full_size(lr, flags, gpu):
     assert_exact(v, i, fv, fi, f"headline lr={lr}")
 
 
+# The kernels the committed MI355X picks run for the three headline batches.
+HEADLINE_KERNELS = {-1: ["k_query_edges_lds<16>"], -2: ["k_query_edges<4, false>"], -3: ["k_query_edges<8, true>"]}
+
+
 @pytest.mark.parametrize("lr", [-1, -2, -3])
 def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
     """The configurations the headline runs: n=1e8, each length distribution
@@ -274,8 +278,11 @@ def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
     cache = tmp_path / "picks.txt"  # a copy: a missing key is tuned here, not written to the committed file
     shutil.copy(os.path.join(os.path.dirname(R.__file__), "tune_cache_mi355x.txt"), cache)
     pick = R.tune(A_dev, Q_dev[: 1 << 22], cache_path=str(cache))
+    assert pick["from_cache"], f"lr={lr}: key {pick['key']} missed the committed cache"
     with R.RMQIndex(A_dev, pick["bs"], flags=pick["flags"]) as ix:
         kern = ix.kernels(pick["opts"])
+    assert kern == HEADLINE_KERNELS[lr], f"lr={lr}: {kern}"
+    with R.RMQIndex(A_dev, pick["bs"], flags=pick["flags"]) as ix:
         v, i = ix.query(Q_dev, opts=pick["opts"])
         torch.cuda.synchronize()
         v, i = v.cpu().numpy(), i.cpu().numpy()
