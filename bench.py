@@ -13,22 +13,31 @@ batch (block size, short-range threshold, lanes per query) from a timed
 distribution (scripts/perf-fulltest.sh:45-58).  value = 3*q*K / (timed
 seconds) of the tuned run; answers are bit-identical either way.
 
-roofline: the dominant kernel of the run -- algorithmic bytes per query
-B = 8 (query) + 4*S (elements scanned: the range for a short or in-block
-query, else both edge-block parts) + T (two 8-byte table entries when whole
-blocks lie between) + 8 (value + index out); with edge tables a long query
-scans nothing and reads two 8-byte edge pairs instead of 4*S.  Summed over
-the queries that kernel processed, divided by its time measured with HIP
-events on the stream it runs on.  `traffic` is the HBM bytes per launch from
-rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/traffic.json, written by
---reduce-pmc), for exactly this workload or null.  The query kernels are
-bound by random reads (sparse-table entries, mostly from L2, and edge pairs
-from DRAM) rather than bytes: `gather_ceiling` sets their counts against
-rmq_probe_gather's random-read rates on the same GPU.
+roofline: the dominant kernel of the run -- algorithmic bytes per query,
+the block method read literally: B = 8 (query) + 4*S (elements scanned: the
+range when it lies inside one block, else both edge-block parts) + T (two
+8-byte table entries when whole blocks lie between) + 8 (value + index
+out), summed over the queries that kernel processed and divided by its time
+measured with HIP events on the stream it runs on.  achieved_model replays
+the kernels' own decisions instead (skipped and exact edges, superset
+tests, edge-table pairs): the bytes this design must move.  `traffic` is the
+HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
+(profiles/traffic.json, written by --reduce-pmc), for exactly this workload
+or null, and per batch measured_bytes_per_rmq / dram_read_lines_per_rmq
+give the same counters per query, comparable across designs.  The query
+kernels are bound by random reads (sparse-table entries, mostly from L2,
+and edge pairs from DRAM) rather than bytes: `gather_ceiling` sets their
+counts against rmq_probe_gather's random-read rates on the same GPU.
+
+stress: n=2^27 tie-heavy and strictly descending arrays, fixed lengths
+1, 2^8, 2^16 and n-l, the tuner's pick against the literal method swept
+over bs = 2^10 .. 2^18, answers compared.
 
 cpu_baseline: the reference's HRMQ (RMQRMM64, built from the reference's own
-sources into oracle/_ref by oracle/Makefile) on this host's cores, OpenMP
-over queries as src/cpu_methods.h:104-110, on a bounded sample of each batch.
+sources into oracle/_ref by oracle/Makefile) on this host's cores, threads
+over queries as src/cpu_methods.h:104-110, on a bounded sample of each batch,
+its answers checked against the GPU's; this build's own HRMQ timed on the
+same sample; CPU BASE (algs 0/100) and HRMQ at n=2^20, q=2^20 beside the GPU.
 """
 from __future__ import annotations
 
@@ -282,6 +291,57 @@ XCDS = 8  # accelerator complex dies of an MI355X (GRBM_GUI_ACTIVE counts each)
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic.json")
 
 
+def literal_bytes(Q, bs: int, out_bytes: int = 8, longer_than: int = 0) -> int:
+    """Algorithmic bytes of a batch under the block method read literally,
+    per RMQ: 8 (the query) + 4*S + T + out_bytes, where S is the range's
+    length when it lies inside one block, else both edge-block parts
+    scanned element by element, (bs - l mod bs) + (r mod bs + 1), and T is
+    16 (two 8-byte table entries) when whole blocks lie between -- what the
+    reference's three traces per query amount to (src/rtx_kernels.cu:85-180).
+    Designs that skip edge parts move fewer bytes than this figure.
+    longer_than: count only ranges of more elements than this."""
+    import torch
+    total = 0
+    for c in range(0, Q.shape[0], 1 << 24):
+        l = Q[c:c + (1 << 24), 0].to(torch.int64)
+        r = Q[c:c + (1 << 24), 1].to(torch.int64)
+        if longer_than:
+            keep = r - l + 1 > longer_than
+            l, r = l[keep], r[keep]
+        lb, rb = l // bs, r // bs
+        S = torch.where(lb == rb, r - l + 1, (bs - l % bs) + (r % bs + 1))
+        T = torch.where(rb - lb > 1, 16, 0)
+        total += int((8 + 4 * S + T + out_bytes).sum().item())
+    return total
+
+
+def load_traffic_record(kernel: str, workload: str):
+    """The whole counter record of `kernel` for exactly this workload, or None."""
+    if not os.path.exists(TRAFFIC_PATH):
+        return None
+    try:
+        with open(TRAFFIC_PATH) as f:
+            return json.load(f).get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def attach_measured(run: dict, workload: str, q: int) -> None:
+    """Per batch, the HBM bytes and DRAM read lines per RMQ its kernels moved
+    (calibrated FETCH_SIZE / WRITE_SIZE of this exact workload, when
+    measured): a figure that stays comparable across designs, unlike either
+    byte model."""
+    for name, d in run["per_dist"].items():
+        recs = [load_traffic_record(sym, workload) for sym in d["kernels"].values()]
+        if not recs or any(r is None for r in recs):
+            continue
+        fetch = sum(r["fetch_bytes_per_launch"] for r in recs)
+        write = sum(r["write_bytes_per_launch"] for r in recs)
+        d["measured_bytes_per_rmq"] = (fetch + write) / q
+        d["dram_read_lines_per_rmq"] = fetch / 128.0 / q
+        d["measured_GBps"] = (fetch + write) / (d["ms_per_pass"] / 1e3) / 1e9
+
+
 def workload_key(n: int, q: int, bs_for: dict, opts_for: dict, edges_for: dict | None) -> str:
     """Names a timed region exactly: HBM traffic measured for one configuration
     is never reported for another."""
@@ -355,12 +415,12 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
         hits = [k for k in per if f"::{sym}(" in k or k.startswith(f"{sym}(")]
         return per[hits[0]] if len(hits) == 1 else None
 
-    out = {}
-    if os.path.exists(TRAFFIC_PATH):
-        with open(TRAFFIC_PATH) as f:
-            out = json.load(f)
     sections = [(line["roofline"]["workload"], line["per_kernel"])]
     sections += [(v["roofline"]["workload"], v["per_kernel"]) for k, v in line.items() if k.startswith("fixed_bs_")]
+    out = {}  # only the workloads this bench line ran: figures for older configurations are dropped
+    if os.path.exists(TRAFFIC_PATH):
+        with open(TRAFFIC_PATH) as f:
+            out = {w: v for w, v in json.load(f).items() if w in {wk for wk, _ in sections}}
     seen = [sym for _, kernels in sections for sym in kernels]
     for workload, kernels in sections:
         for sym, info in kernels.items():
@@ -393,33 +453,88 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
                 "dispatches": len(rec["FETCH_SIZE"]),
                 "algorithmic_bytes_per_launch": info["algorithmic_bytes_per_launch"],
                 "traffic_over_algorithmic": (fb + wb) / info["algorithmic_bytes_per_launch"],
+                "model_bytes_per_launch": info.get("model_bytes_per_launch"),
+                "traffic_over_model": ((fb + wb) / info["model_bytes_per_launch"]
+                                       if info.get("model_bytes_per_launch") else None),
+                "queries_per_launch": info.get("queries_per_launch"),
                 "calibration": {"FETCH_SIZE": cal["FETCH_SIZE"], "WRITE_SIZE": cal["WRITE_SIZE"],
                                 "copy_bytes": copy_bytes}}
             log(f"traffic: {sym}: {(fb + wb) / 1e6:.1f} MB/launch over {len(rec['FETCH_SIZE'])} dispatches, "
-                f"{(fb + wb) / info['algorithmic_bytes_per_launch']:.2f}x algorithmic")
+                f"{(fb + wb) / info['algorithmic_bytes_per_launch']:.2f}x the literal bytes, "
+                f"{(fb + wb) / info['model_bytes_per_launch']:.2f}x the model's")
     with open(TRAFFIC_PATH, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
     return 0
 
 
-def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | None:
-    """Reference HRMQ on the host cores over a bounded sample of each batch."""
+def host_info() -> dict:
+    """CPU model, the cores this process may run on, and SMT, of the host."""
+    model, siblings, cores = "unknown", None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "siblings" and siblings is None:
+                    siblings = int(v)
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        if siblings and cores:
+            smt = siblings > cores
+    return {"cpu_model": model, "affinity_cores": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "smt_active": smt, "threads_per_core": (siblings // cores) if siblings and cores else None}
+
+
+def compare_hrmq(A_host: np.ndarray, val_g, idx_g, val_h, idx_h) -> dict:
+    """GPU answers (leftmost index) against HRMQ's (rightmost index) on the
+    same queries: values bit for bit; indices equal where the minimum is
+    unique, and where they differ both must hold the minimum with HRMQ's
+    further right (a tie)."""
+    vb = val_g.view(np.int32) != val_h.view(np.int32)
+    differ = idx_g != idx_h
+    tie_ok = differ & (idx_h > idx_g) & (A_host.view(np.int32)[idx_h] == val_g.view(np.int32))
+    return {"compared": int(val_g.size), "value_mismatches": int(vb.sum()),
+            "index_ties": int(tie_ok.sum()), "index_mismatches": int((differ & ~tie_ok).sum())}
+
+
+def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers: dict,
+                 config1: dict | None) -> dict | None:
+    """The reference's HRMQ on this host's cores over a bounded sample of each
+    batch, checked against the GPU's answers to the same queries; this
+    build's own HRMQ (algs 1/101 of its CLI) on the same sample beside it;
+    and CPU BASE (algs 0/100) at the small configuration."""
     try:
         from oracle import oracle as O
     except Exception as e:  # noqa: BLE001
         log(f"cpu_baseline unavailable: {e}")
         return None
-    nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    import rtxrmq_amd as R
+    info = host_info()
+    nt = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["affinity_cores"]
     if not O.HRMQ.available():
         log("cpu_baseline: oracle/_ref/libhrmq.so missing; skipping")
         return None
-    log(f"cpu_baseline: building reference HRMQ over n={A_host.size} (single thread)")
+    log(f"cpu_baseline: {info['cpu_model']}, {info['affinity_cores']} cores in affinity, SMT {info['smt_active']}; "
+        f"building reference HRMQ over n={A_host.size} (single thread)")
     t0 = time.perf_counter()
     h = O.HRMQ(A_host)
     build_s = time.perf_counter() - t0
-    log(f"cpu_baseline: built in {build_s:.2f} s; timing queries on {nt} threads")
-    total_q, total_t, per = 0, 0.0, {}
+    t0 = time.perf_counter()
+    port = R.HostHRMQ(A_host)
+    port_build_s = time.perf_counter() - t0
+    log(f"cpu_baseline: reference built in {build_s:.2f} s, this build's HRMQ in {port_build_s:.2f} s; "
+        f"timing queries on {nt} threads")
+    total_q, total_t, port_t, per = 0, 0.0, 0.0, {}
     for lr, Q in queries.items():
         pilot = Q[: 1 << 16]
         t0 = time.perf_counter()
@@ -428,14 +543,54 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
         m = int(min(Q.shape[0], max(1 << 16, rate * target_s)))
         sample = Q[:m]
         t0 = time.perf_counter()
-        h.query(sample, nt=nt, want_val=True, want_idx=True)
+        hv, hi = h.query(sample, nt=nt, want_val=True, want_idx=True)
         dt = time.perf_counter() - t0
-        per[DIST_NAMES[lr]] = {"rmq_per_s": m / dt, "queries": m}
-        log(f"cpu_baseline: {DIST_NAMES[lr]} {m / dt:.3e} RMQ/s over {m} queries")
+        t0 = time.perf_counter()
+        pv, pi = port.query(sample, nthreads=nt)
+        pdt = time.perf_counter() - t0
+        gv, gi = gpu_answers[lr]
+        cmp = compare_hrmq(A_host, gv[:m], gi[:m], hv, hi)
+        port_bad = int(((pi != hi) | (pv.view(np.int32) != hv.view(np.int32))).sum())
+        per[DIST_NAMES[lr]] = {"rmq_per_s": m / dt, "queries": m, "port_rmq_per_s": m / pdt,
+                               "gpu_vs_hrmq": cmp, "port_vs_reference_mismatches": port_bad}
+        log(f"cpu_baseline: {DIST_NAMES[lr]} reference {m / dt:.3e} RMQ/s, this build's {m / pdt:.3e} RMQ/s over "
+            f"{m} queries; GPU vs HRMQ: {cmp}; port vs reference mismatches {port_bad}")
+        if cmp["value_mismatches"] or cmp["index_mismatches"] or port_bad:
+            raise SystemExit(f"cpu_baseline: answers differ on {DIST_NAMES[lr]}")
         total_q += m
         total_t += dt
-    size = h.size_bytes()
+        port_t += pdt
+    size, port_size = h.size_bytes(), port.bytes()
     h.close()
+    port.close()
+    small = None
+    if config1:
+        # CPU BASE (brute force from A[l], strict '<'), bounded sample; HRMQ on every query
+        A1, Q1 = config1["A"], config1["Q"]
+        pilot = Q1[:64]
+        t0 = time.perf_counter()
+        O.rmq_brute(A1, pilot, nt=nt)
+        rate = pilot.shape[0] / max(time.perf_counter() - t0, 1e-6)
+        m = int(min(Q1.shape[0], max(256, rate * target_s)))
+        t0 = time.perf_counter()
+        bv, bi = O.rmq_brute(A1, Q1[:m], nt=nt)
+        base_dt = time.perf_counter() - t0
+        g_v, g_i = config1["val"], config1["idx"]
+        base_bad = int(((bi != g_i[:m]) | (bv.view(np.int32) != g_v[:m].view(np.int32))).sum())
+        h1 = O.HRMQ(A1)
+        t0 = time.perf_counter()
+        hv, hi = h1.query(Q1, nt=nt, want_val=True, want_idx=True)
+        h1_dt = time.perf_counter() - t0
+        h1.close()
+        cmp = compare_hrmq(A1, g_v, g_i, hv, hi)
+        small = {"n": int(A1.size), "q": int(Q1.shape[0]), "lr": -1,
+                 "gpu_rmq_per_s": config1["rmq_per_s"], "gpu_bs": config1["bs"],
+                 "cpu_base_rmq_per_s": m / base_dt, "cpu_base_queries": m, "cpu_base_mismatches": base_bad,
+                 "hrmq_rmq_per_s": Q1.shape[0] / h1_dt, "gpu_vs_hrmq": cmp}
+        log(f"cpu_baseline: config 1 GPU {config1['rmq_per_s']:.3e} RMQ/s, CPU BASE {m / base_dt:.3e} RMQ/s "
+            f"({m} queries, {base_bad} mismatches), HRMQ {Q1.shape[0] / h1_dt:.3e} RMQ/s, GPU vs HRMQ {cmp}")
+        if base_bad or cmp["value_mismatches"] or cmp["index_mismatches"]:
+            raise SystemExit("cpu_baseline: config 1 answers differ")
     return {
         "value": total_q / total_t,
         "unit": "RMQ/s",
@@ -443,8 +598,14 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float) -> dict | N
         "kind": "reference",
         "sample": (f"reference HRMQ (RMQRMM64, alg 1/101 value+index) on the first {total_q} queries of the same "
                    f"three batches (about {target_s:.0f}s each on {nt} threads); construction {build_s:.2f}s "
-                   f"single-threaded, {size / 1e6:.1f} MB"),
+                   f"single-threaded, {size / 1e6:.1f} MB; answers checked against the GPU's"),
+        "host": info,
+        "port_rmq_per_s": total_q / port_t,
+        "port_over_reference": total_t / port_t,
+        "port_construction_s": port_build_s,
+        "port_bytes": port_size,
         "per_dist": per,
+        "config1": small,
     }
 
 
@@ -487,6 +648,7 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
                                sub=levels.get(32) if e else None, A=A, lines=lines, two_stage=two)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
+        lit = literal_bytes(Q_dev[lr], bs_for[lr])
         ms = [s[lr]["total_ms"] for s in steps_out]
         lms = [s[lr]["long_ms"] for s in steps_out]
         sms = [s[lr]["short_ms"] for s in steps_out]
@@ -502,29 +664,40 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             "edge_parts_computed": ab["edge_parts_computed"],
             "algorithmic_GBps": (lb + sb) / (np.mean(ms) / 1e3) / 1e9,
             "frac_of_roofline": (lb + sb) / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "model_bytes_per_rmq": (lb + sb) / q,
+            "literal_bytes_per_rmq": lit / q,
+            "literal_frac_of_roofline": lit / (np.mean(ms) / 1e3) / 1e9 / HBM_PEAK_GBS,
         }
         if e:  # one pass answers every range (G lanes per scanned range)
             sym = kernels_for[lr][0] if kernels_for else f"k_query_edges<{opts_for[lr].group_lanes}>"
             per_dist[DIST_NAMES[lr]]["kernels"] = {"all": sym}
-            launches = ((sym, lb + sb, lms, ab["gathers"], ab["table_reads"], ab["short_lines"]),)
+            launches = ((sym, lb + sb, lms, ab["gathers"], ab["table_reads"], ab["short_lines"], lit, q),)
         else:
             long_sym = f"k_query_tiles<{opts_for[lr].long_lanes}, true>"
             short_sym = f"k_query_tiles<{opts_for[lr].group_lanes}, false>"
             per_dist[DIST_NAMES[lr]]["kernels"] = {"long": long_sym, "short": short_sym}
-            launches = ((long_sym, lb, lms, 0, ab["table_reads"], 0), (short_sym, sb, sms, 0, 0, 0))
-        for sym, b, t, g, rd, ln in launches:
-            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0, 0, 0])
+            lit_long = literal_bytes(Q_dev[lr], bs_for[lr], longer_than=opts_for[lr].short_max)
+            launches = ((long_sym, lb, lms, 0, ab["table_reads"], 0, lit_long, nl),
+                        (short_sym, sb, sms, 0, 0, 0, lit - lit_long, q - nl))
+        for sym, b, t, g, rd, ln, lit_b, nq in launches:
+            acc = kernels.setdefault(sym, [0.0, 0.0, 0, 0, 0, 0, 0.0, 0])
             acc[0] += b * len(steps_out)
             acc[1] += float(np.sum(t))
             acc[2] += len(steps_out)
             acc[3] += g * len(steps_out)
             acc[4] += rd * len(steps_out)
             acc[5] += ln * len(steps_out)
+            acc[6] += lit_b * len(steps_out)
+            acc[7] += nq * len(steps_out)
     dom = max(kernels, key=lambda k: kernels[k][1])
-    achieved = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
+    achieved = kernels[dom][6] / (kernels[dom][1] / 1e3) / 1e9
+    achieved_model = kernels[dom][0] / (kernels[dom][1] / 1e3) / 1e9
     per_kernel = {k: {"ms": v[1], "launches": v[2], "avg_launch_ms": v[1] / v[2],
-                      "algorithmic_bytes_per_launch": v[0] / v[2],
-                      "achieved_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
+                      "queries_per_launch": v[7] / v[2],
+                      "algorithmic_bytes_per_launch": v[6] / v[2],
+                      "achieved_GBps": v[6] / max(v[1], 1e-9) * 1e3 / 1e9,
+                      "model_bytes_per_launch": v[0] / v[2],
+                      "model_GBps": v[0] / max(v[1], 1e-9) * 1e3 / 1e9} for k, v in kernels.items()}
     for k, v in kernels.items():
         if k.startswith("k_query_edges_lds"):
             per_kernel[k]["table_in_lds"] = True
@@ -535,7 +708,8 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             per_kernel[k]["table_reads_per_s"] = v[4] / (v[1] / 1e3)
             per_kernel[k]["short_lines_per_launch"] = v[5] / v[2]
     return {"value": 3 * q * len(steps_out) / elapsed, "ms_per_step": elapsed / len(steps_out) * 1e3,
-            "per_dist": per_dist, "dominant": dom, "achieved": achieved, "per_kernel": per_kernel}
+            "per_dist": per_dist, "dominant": dom, "achieved": achieved, "achieved_model": achieved_model,
+            "per_kernel": per_kernel}
 
 
 def run_batched(R, torch, dev, args) -> dict:
@@ -745,12 +919,14 @@ def run_served(R, torch, dev, args) -> dict:
 def run_stress(R, torch, dev, args) -> dict:
     """Config 3: n=2^27 arrays of heavy ties (values in {0..15}) and strictly
     descending values, fixed range lengths 1, 2^8, 2^16 and n-l (every range
-    ends at n-1); per workload the autotuner's pick (tuned afresh on 2^24 of
-    the queries, no cache) and its RMQ/s, and the literal method (no edge
-    tables, default lanes) swept over bs = 2^10 .. 2^18.  Every swept
-    configuration's answers are compared in full with the tuned ones."""
+    ends at n-1); per workload the autotuner's pick (from the tuner cache, or
+    timed on a separately seeded stream of the same workload) and its RMQ/s,
+    and the literal method (no edge tables, default lanes) swept over
+    bs = 2^10 .. 2^18.  Every swept configuration's answers are compared in
+    full with the tuned ones."""
     n, q = 1 << 27, args.stress_q
     sample = min(q, args.tune_sample)
+    steps = max(1, args.stress_steps)
     out = {}
     v_t = torch.empty(q, dtype=torch.float32, device=dev)
     i_t = torch.empty(q, dtype=torch.int32, device=dev)
@@ -759,14 +935,15 @@ def run_stress(R, torch, dev, args) -> dict:
 
     def rate(ix, Q, opts, v, i):
         ix.query(Q, opts=opts, out_val=v, out_idx=i)
-        ms = [ix.query_timed(Q, opts=opts, out_val=v, out_idx=i)[2]["total_ms"] for _ in range(args.steps)]
+        ms = [ix.query_timed(Q, opts=opts, out_val=v, out_idx=i)[2]["total_ms"] for _ in range(steps)]
         return sum(ms) / len(ms)
 
     for kind in ("ties16", "desc"):
         A = R.gen_array(n, 99, kind, device=dev)
         for lr in (1, 1 << 8, 1 << 16, -6):
             Q = R.gen_queries_device(q, n, lr, args.seed)
-            pick = R.tune(A, Q[:sample], cache_path=None)
+            pick = R.tune(A, R.gen_queries_device(sample, n, lr, args.seed + 7919), cache_path=args.tune_cache or None,
+                          refresh=args.retune)
             with R.RMQIndex(A, pick["bs"], flags=pick["flags"]) as ix:
                 ms_t = rate(ix, Q, pick["opts"], v_t, i_t)
                 kern = ix.kernels(pick["opts"])
@@ -780,6 +957,7 @@ def run_stress(R, torch, dev, args) -> dict:
             name = {1: "len 1", 256: "len 2^8", 65536: "len 2^16", -6: "len n-l"}[lr]
             out.setdefault(kind, {})[name] = {
                 "tuned": {"bs": pick["bs"], "edges": bool(pick["flags"]), "short_max": o.short_max,
+                          "source": "cache" if pick["from_cache"] else "timed",
                           "group_lanes": o.group_lanes, "long_lanes": o.long_lanes, "kernels": kern,
                           "ms_per_pass": ms_t, "rmq_per_s": q / (ms_t * 1e-3)},
                 "bs_sweep_rmq_per_s": sweep,
@@ -793,7 +971,25 @@ def run_stress(R, torch, dev, args) -> dict:
             del Q
         del A
         torch.cuda.empty_cache()
-    return {"n": n, "q": q, "tune_sample": sample, "workloads": out}
+    return {"n": n, "q": q, "tune_sample": sample, "steps": steps, "workloads": out}
+
+
+def run_config1(R, torch, dev, args) -> dict:
+    """The small configuration: n=2^20 U[0,1), 2^20 uniform-length queries,
+    the literal bs=2^15, values and indices, timed on the GPU; its arrays
+    and answers go to the host checks of the CPU baseline."""
+    n1, q1, bs1 = 1 << 20, 1 << 20, 1 << 15
+    A1 = R.gen_array(n1, args.seed, "uniform", device=dev)
+    Q1 = R.gen_queries(q1, n1, -1, args.seed)
+    Qd = torch.from_numpy(Q1).to(dev)
+    with R.RMQIndex(A1, bs1) as ix:
+        v, i = ix.query(Qd)
+        ms = [ix.query_timed(Qd, out_val=v, out_idx=i)[2]["total_ms"] for _ in range(5)]
+        torch.cuda.synchronize()
+        out = {"A": A1.cpu().numpy(), "Q": Q1, "val": v.cpu().numpy(), "idx": i.cpu().numpy(),
+               "rmq_per_s": q1 / (float(np.mean(ms)) / 1e3), "bs": bs1}
+    log(f"config 1: n={n1} q={q1} bs={bs1}: {out['rmq_per_s']:.3e} RMQ/s")
+    return out
 
 
 def main() -> int:
@@ -825,7 +1021,9 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered rate / saturation rate")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--stress-q", type=int, default=1 << 26, help="queries per workload of --only stress")
+    ap.add_argument("--stress-q", type=int, default=1 << 24, help="queries per workload of the stress leg")
+    ap.add_argument("--stress-steps", type=int, default=2, help="timed passes per configuration of the stress leg")
+    ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--only", choices=["batched", "served", "stress"], default=None,
                     help="run one section alone and print its JSON object (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -914,6 +1112,12 @@ def main() -> int:
     bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
     steps_out, elapsed = timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, args.steps, args.warmup,
                                    "tuned")
+    gpu_answers = {}  # the tuned answers, checked against HRMQ on the host after the timed region
+    if not args.no_cpu_baseline:
+        for lr in DISTS:
+            index_for[lr].query(Q_dev[lr], out_val=out_v, out_idx=out_i, opts=opts_for[lr])
+            torch.cuda.synchronize()
+            gpu_answers[lr] = (out_v.cpu().numpy(), out_i.cpu().numpy())
     for b in (set(bs_for.values()) | ({32} if any(p["flags"] for p in picks.values()) else set())) - set(levels):
         levels[b] = BlockLevel(A, b)
     kernels_for = {lr: index_for[lr].kernels(opts_for[lr]) for lr in DISTS}
@@ -948,11 +1152,15 @@ def main() -> int:
     del out_v, out_i
     batched = None if args.no_batched else run_batched(R, torch, dev, args)
     served = None if args.no_served else run_served(R, torch, dev, args)
+    stress = None if args.no_stress else run_stress(R, torch, dev, args)
     cpu = None
     del levels
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds)
+        config1 = run_config1(R, torch, dev, args)
+        cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds, gpu_answers, config1)
 
+    attach_measured(tuned, workload, q)
+    attach_measured(fixed, fixed_workload, q)
     traffic = load_traffic(tuned["dominant"], workload)
     dom_ms = tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"]
     fixed_traffic = load_traffic(fixed["dominant"], fixed_workload)
@@ -980,6 +1188,9 @@ def main() -> int:
                    "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},
         "roofline": {"bound": "hbm", "achieved": tuned["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tuned["achieved"] / HBM_PEAK_GBS,
+                     "bytes_model": "literal: 8 + 4*S + T + 8 per RMQ (both edge parts scanned); "
+                                    "achieved_model: the kernels' own decisions replayed",
+                     "achieved_model": tuned["achieved_model"], "frac_model": tuned["achieved_model"] / HBM_PEAK_GBS,
                      "traffic": traffic, "kernel": tuned["dominant"],
                      # the HBM bytes the counters saw, over the same launch time: random
                      # 8-byte reads each move a whole 128-byte DRAM request
@@ -992,6 +1203,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "stress": stress,
         "per_dist": tuned["per_dist"],
         "per_kernel": tuned["per_kernel"],
         f"fixed_bs_{args.bs}": {
@@ -1000,6 +1212,8 @@ def main() -> int:
             "per_kernel": fixed["per_kernel"],
             "roofline": {"kernel": fixed["dominant"], "achieved": fixed["achieved"],
                          "frac": fixed["achieved"] / HBM_PEAK_GBS,
+                         "achieved_model": fixed["achieved_model"],
+                         "frac_model": fixed["achieved_model"] / HBM_PEAK_GBS,
                          "traffic": fixed_traffic,
                          "traffic_GBps": fixed_traffic / (fixed_ms / 1e3) / 1e9 if fixed_traffic else None,
                          "avg_launch_ms": fixed_ms,

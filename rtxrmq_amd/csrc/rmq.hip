@@ -342,6 +342,22 @@ __global__ void k_gather_stride(const int2 *__restrict__ Q, int64_t step, int64_
     if (k < m) out[k] = Q[k * step];
 }
 
+// Value signature of an array for the tuner's cache key: one wave per
+// window of w <= 64 elements starting at k * step; writes the leftmost
+// minimum's offset in the window and how many elements equal that minimum.
+__global__ __launch_bounds__(kWG) void k_window_sig(const float *__restrict__ A, int w, int64_t step, int64_t m,
+                                                    int2 *__restrict__ out) {
+    const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // uniform across the wave
+    if (k >= m) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const float x = lane < w ? A[k * step + lane] : INFINITY;
+    float v = x;
+    int i = lane < w ? lane : INT_MAX;
+    wave_reduce(v, i);
+    const uint64_t eq = __ballot(lane < w && x == v);
+    if (lane == 0) out[k] = make_int2(i, __popcll(eq));
+}
+
 __global__ void k_count_bad(const int2 *__restrict__ Q, int64_t q, int64_t n, unsigned long long *bad) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool b = false;
@@ -947,21 +963,20 @@ constexpr int kSigWindows = 4096;  // windows sampled, evenly spaced
 int value_signature(const float *d_A, int64_t n, hipStream_t s, char *out, size_t out_len) {
     const int w = (int)std::min<int64_t>(n, kSigWindow);
     const int64_t m = std::min<int64_t>(kSigWindows, n / w);
-    const int64_t step = (n - w) / std::max<int64_t>(1, m - 1);
-    std::vector<float> h((size_t)(m * w));
-    // window k starts at k * step (step >= w), the last one ends at or before n
-    RMQ_HIP(hipMemcpy2DAsync(h.data(), sizeof(float) * (size_t)w, d_A, sizeof(float) * (size_t)std::max<int64_t>(step, w),
-                             sizeof(float) * (size_t)w, (size_t)m, hipMemcpyDeviceToHost, s));
-    RMQ_HIP(hipStreamSynchronize(s));
+    const int64_t step = (n - w) / std::max<int64_t>(1, m - 1);  // window k starts at k * step, ends by n
+    int2 *d_sig = nullptr;
+    RMQ_HIP(hipMalloc(&d_sig, sizeof(int2) * (size_t)m));
+    std::vector<int2> sig((size_t)m);
+    k_window_sig<<<(unsigned)((m + kWavesPerWG - 1) / kWavesPerWG), kWG, 0, s>>>(d_A, w, step, m, d_sig);
+    int rc = hip_rc(hipGetLastError());
+    if (rc == RMQ_OK) rc = hip_rc(hipMemcpyAsync(sig.data(), d_sig, sizeof(int2) * (size_t)m, hipMemcpyDeviceToHost, s));
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    (void)hipFree(d_sig);
+    if (rc != RMQ_OK) return rc;
     double pos = 0.0, extra = 0.0;
-    for (int64_t k = 0; k < m; ++k) {
-        const float *x = h.data() + k * w;
-        int am = 0, eq = 0;
-        for (int j = 1; j < w; ++j)
-            if (x[j] < x[am]) am = j;
-        for (int j = 0; j < w; ++j) eq += x[j] == x[am];
-        pos += w > 1 ? (double)am / (w - 1) : 0.5;
-        extra += eq - 1;
+    for (const int2 &g : sig) {
+        pos += w > 1 ? (double)g.x / (w - 1) : 0.5;
+        extra += g.y - 1;
     }
     pos /= (double)m;
     extra /= (double)m;
