@@ -330,8 +330,15 @@ def attach_measured(run: dict, workload: str, q: int) -> None:
     """Per batch, the HBM bytes and DRAM read lines per RMQ its kernels moved
     (calibrated FETCH_SIZE / WRITE_SIZE of this exact workload, when
     measured): a figure that stays comparable across designs, unlike either
-    byte model."""
+    byte model.  Only for batches whose kernels no other batch launches:
+    the counters of a kernel shared by several batches mix them."""
+    users = {}
+    for d in run["per_dist"].values():
+        for sym in d["kernels"].values():
+            users[sym] = users.get(sym, 0) + 1
     for name, d in run["per_dist"].items():
+        if any(users[sym] > 1 for sym in d["kernels"].values()):
+            continue
         recs = [load_traffic_record(sym, workload) for sym in d["kernels"].values()]
         if not recs or any(r is None for r in recs):
             continue
