@@ -756,7 +756,7 @@ def run_batched(R, torch, dev, args) -> dict:
     torch.cuda.synchronize()
     big = int(np.argmax(ns))
     pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096],
-                  cache_path=args.tune_cache or None, refresh=args.retune)
+                  cache_path=args.tune_cache or None, refresh=args.retune, want_value=False)
     opts, bs, flags = pick["opts"], pick["bs"], pick["flags"]
     log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} edges={bool(flags)} "
         f"short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
@@ -822,7 +822,7 @@ def run_served(R, torch, dev, args) -> dict:
     big = int(np.argmax(ns))
     sample = torch.cat([R.gen_queries_device(1 << 20, int(ns[big]), lr, 77, device=dev) for lr in (-1, -7, -8)])
     pick = R.tune(arrays[big], sample, bs_list=[16, 32, 64, 128, 256, 1024], cache_path=args.tune_cache or None,
-                  refresh=args.retune)
+                  refresh=args.retune, want_value=False)
     bs, opts, flags = pick["bs"], pick["opts"], pick["flags"]
     log(f"served: bs={bs} edges={bool(flags)} short_max={opts.short_max} group_lanes={opts.group_lanes} "
         f"long_lanes={opts.long_lanes}")
