@@ -798,12 +798,72 @@ def run_batched(R, torch, dev, args) -> dict:
     plan.close()
     for ix in indexes:
         ix.close()
+    per_class = run_batched_classes(R, torch, args, ns, A_views, Q_views, q_off, out_loop, timed)
     return {"arrays": m, "elements": int(ns.sum()), "queries": total_q, "bs": bs, "edge_tables": bool(flags),
+            "per_class": per_class,
             "opts": {"short_max": opts.short_max, "group_lanes": opts.group_lanes, "long_lanes": opts.long_lanes},
             "work_items": info["items"], "rmq_per_s_graph": total_q / t_graph, "rmq_per_s_plan": total_q / t_eager,
             "rmq_per_s_loop": total_q / t_loop, "speedup_vs_loop": t_loop / t_graph,
             "ms_graph": t_graph * 1e3, "ms_loop": t_loop * 1e3, "build_s": build_s, "plan_create_s": plan_s,
             "steps": args.batch_steps, "identical_to_loop": identical}
+
+
+def run_batched_classes(R, torch, args, ns, A_views, Q_views, q_off, out_ref, timed) -> dict:
+    """The batched workload with picks per size class: arrays of the same
+    floor(log2 n) share the block size and edge tables the tuner picks on the
+    class's largest array (index outputs, cached per key); the plan's
+    thresholds and lanes are those of the class holding the most queries.
+    Timed as one graph launch and as the per-array loop on the same handles;
+    answers compared with the single-pick loop's (out_ref)."""
+    import ctypes
+    from rtxrmq_amd import _native as N
+    m = len(A_views)
+    total_q = int(q_off[-1])
+    cls = np.floor(np.log2(ns)).astype(np.int64)
+    picks = {}
+    for c in sorted(set(cls.tolist())):
+        members = np.nonzero(cls == c)[0]
+        rep = int(members[np.argmax(ns[members])])
+        picks[c] = R.tune(A_views[rep], Q_views[rep], bs_list=[16, 32, 64, 128, 256, 512, 1024, 4096],
+                          cache_path=args.tune_cache or None, refresh=args.retune, want_value=False)
+    q_per = {c: int(ns[cls == c].sum()) for c in picks}
+    top = max(q_per, key=q_per.get)
+    opts = picks[top]["opts"]
+    indexes = [R.RMQIndex(A_views[i], picks[int(cls[i])]["bs"], flags=picks[int(cls[i])]["flags"]) for i in range(m)]
+    torch.cuda.synchronize()
+    out = torch.empty_like(out_ref)
+    outs = [out[q_off[i]:q_off[i + 1]] for i in range(m)]
+    plan = R.Plan(indexes, Q_views, out_idx=outs, opts=opts)
+    P = ctypes.c_void_p * m
+    hs = P(*[ix.handle for ix in indexes])
+    qp = P(*[Q.data_ptr() for Q in Q_views])
+    qs = (ctypes.c_int64 * m)(*[int(x) for x in ns])
+    ip = P(*[t.data_ptr() for t in outs])
+
+    def loop():
+        N.check(N.lib().rmq_query_loop(hs, m, qp, qs, None, ip, ctypes.byref(opts), None), "rmq_query_loop")
+
+    t_graph = timed(plan.graph_launch, args.batch_steps)
+    identical_graph = bool(torch.equal(out, out_ref))
+    t_loop = timed(loop, args.batch_steps)
+    identical_loop = bool(torch.equal(out, out_ref))
+    plan.close()
+    for ix in indexes:
+        ix.close()
+    del out
+    classes = {f"2^{c}": {"arrays": int((cls == c).sum()), "queries": q_per[c], "bs": picks[c]["bs"],
+                          "edge_tables": bool(picks[c]["flags"]), "short_max": picks[c]["opts"].short_max,
+                          "group_lanes": picks[c]["opts"].group_lanes, "long_lanes": picks[c]["opts"].long_lanes,
+                          "source": "cache" if picks[c]["from_cache"] else "timed"} for c in picks}
+    log(f"batched per class: graph {total_q / t_graph:.4e} RMQ/s ({t_graph * 1e3:.1f} ms), loop "
+        f"{total_q / t_loop:.4e} ({t_loop * 1e3:.1f} ms); plan opts from class 2^{top}; "
+        f"identical={identical_graph and identical_loop}; picks " +
+        ", ".join(f"{k}:bs={v['bs']}{'+e' if v['edge_tables'] else ''}" for k, v in classes.items()))
+    if not (identical_graph and identical_loop):
+        raise SystemExit("batched per class: answers differ from the single-pick loop")
+    return {"rmq_per_s_graph": total_q / t_graph, "rmq_per_s_loop": total_q / t_loop,
+            "speedup_vs_loop": t_loop / t_graph, "ms_graph": t_graph * 1e3, "ms_loop": t_loop * 1e3,
+            "plan_opts_from_class": f"2^{top}", "classes": classes, "identical_to_single_pick": True}
 
 
 def run_served(R, torch, dev, args) -> dict:
@@ -909,6 +969,7 @@ def run_served(R, torch, dev, args) -> dict:
     offered = args.serve_load * req_rate
     poi, poi_rate, poi_bad = run_stream(args.serve_requests, offered, f"poisson {offered:.0f} req/s")
     sv.close()
+    staged = run_staged(R, args, arrays, bs, flags, opts, payloads, plain_out, rng)
     return {"arrays": m, "elements": int(ns.sum()), "bs": bs, "edge_tables": bool(flags), "slots": args.serve_slots,
             "request_q": "log-uniform [2^8, 2^22]", "mean_q": float(pq.mean()), "payloads": len(payloads),
             "plain_rmq_per_s": plain_rate, "plain_requests": len(order),
@@ -918,9 +979,43 @@ def run_served(R, torch, dev, args) -> dict:
                         "rmq_per_s": poi_rate, "requests": poi["requests_done"], "p50_ms": poi["p50_ms"],
                         "p99_ms": poi["p99_ms"], "mean_ms": poi["mean_ms"], "max_ms": poi["max_ms"],
                         "requests_per_launch": poi["mean_requests_per_launch"]},
-            "speedup_vs_plain": sat_rate / plain_rate, "mismatches_vs_plain": sat_bad + poi_bad,
+            "staged_under_budget": staged,
+            "speedup_vs_plain": sat_rate / plain_rate, "mismatches_vs_plain": sat_bad + poi_bad + staged["mismatches"],
             "budget_bytes": sat["budget_bytes"], "peak_inflight_bytes": max(sat["peak_inflight_bytes"],
                                                                             poi["peak_inflight_bytes"])}
+
+
+def run_staged(R, args, arrays, bs, flags, opts, payloads, plain_out, rng) -> dict:
+    """Requests in pageable host memory, staged through device memory, under
+    a fixed budget (--serve-budget) far below what all of them need at once:
+    the server admits them as launches in flight free their staging buffers.
+    All are submitted at once; answers compared with the plain loop's."""
+    budget = args.serve_budget
+    with R.Server(slots=args.serve_slots, opts=opts, mem_budget_bytes=budget) as sv:
+        ids = [sv.add_array(A, bs, flags) for A in arrays]
+        choice = rng.integers(0, len(payloads), args.serve_staged_requests)
+        Qs = [payloads[int(c)][1].numpy().copy() for c in choice]  # pageable copies
+        outs = [np.empty(Q.shape[0], np.int32) for Q in Qs]
+        need = sum(12 * Q.shape[0] for Q in Qs)
+        sv.stats(reset=True)
+        tickets = [sv.submit(ids[payloads[int(c)][0]], Q, out_idx=o) for c, Q, o in zip(choice, Qs, outs)]
+        failed = sum(sv.wait(t)[0] != R.RMQ_OK for t in tickets)
+        st = sv.stats()
+    mismatches = failed
+    for c, o in zip(choice, outs):
+        ref = plain_out.get(int(c))
+        if ref is not None and not np.array_equal(o, ref.numpy()):
+            mismatches += 1
+    rate = st["queries_done"] / (st["wall_ms"] / 1e3)
+    log(f"served[staged, budget {budget / 2**20:.0f} MiB]: {st['requests_done']} requests needing "
+        f"{need / 2**20:.0f} MiB at once, peak in flight {st['peak_inflight_bytes'] / 2**20:.1f} MiB, "
+        f"{rate:.4e} RMQ/s, p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, "
+        f"{st['mean_requests_per_launch']:.1f} requests/launch; mismatches {mismatches}")
+    if st["peak_inflight_bytes"] > budget:
+        raise SystemExit("served: staged requests exceeded the memory budget")
+    return {"budget_bytes": budget, "requests": st["requests_done"], "bytes_needed_at_once": need,
+            "peak_inflight_bytes": st["peak_inflight_bytes"], "rmq_per_s": rate, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "requests_per_launch": st["mean_requests_per_launch"], "mismatches": mismatches}
 
 
 def run_stress(R, torch, dev, args) -> dict:
@@ -1028,6 +1123,8 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered rate / saturation rate")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-budget", type=int, default=64 << 20, help="device budget of the staged served run")
+    ap.add_argument("--serve-staged-requests", type=int, default=400)
     ap.add_argument("--stress-q", type=int, default=1 << 24, help="queries per workload of the stress leg")
     ap.add_argument("--stress-steps", type=int, default=2, help="timed passes per configuration of the stress leg")
     ap.add_argument("--no-stress", action="store_true")
