@@ -129,9 +129,10 @@ int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int64_t *nb,
 int rmq_handle_bytes(int64_t n, int32_t bs, uint32_t flags, int64_t *bytes);
 
 /* Names of the kernels rmq_query launches on this handle with these options
- * (NULL = defaults), separated by ';' in launch order, as a profiler reports
- * them: "k_query_edges_lds<G>" (edge tables, block sparse table held in
- * LDS), "k_query_edges<G, TWO>" (edge tables; TWO = true when the table is
+ * (NULL = defaults) for a query that asks for both outputs, separated by ';'
+ * in launch order, as a profiler reports them: "k_query_edges_lds<G, 3>"
+ * (edge tables, block sparse table held in LDS; a query asking for values
+ * only runs <G, 1>, for indexes only <G, 2>), "k_query_edges<G, TWO>" (edge tables; TWO = true when the table is
  * large enough that failed superset tests are retried) or
  * "k_query_tiles<G, false>;k_query_tiles<GL, true>".  Writes a NUL-terminated string of at most len
  * bytes into buf; RMQ_ERR_INVALID_ARG when it does not fit.  No counterpart

@@ -257,40 +257,76 @@ __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, E
     }
 }
 
-// The same pass with the block sparse table copied into LDS first, for a
-// table that fits: the superset test and the edge blocks' (min, argmin) then
-// read LDS, and the vector-memory path (whose address and L1 stages bound
-// the pass above: two divergent table reads per query) carries only the
-// queries, the results, edge pairs and scans.  Workgroups of kLdsWaves waves,
-// two resident per CU.
+// One tile of a handle with edge tables, its block sparse table in LDS: the
+// answer of this lane's query in registers -- (NaN, -1) for an invalid query,
+// the O(1) edge-table answer for a long range, and for a scanned range the
+// result its lane group leaves in the wave's LDS slots.  Writes nothing to
+// global memory.
 template <int G>
+__device__ __forceinline__ Pair answer_tile_lds(const Blocks &kl, const EdgeTables &e, int2 p, int64_t q, int64_t t,
+                                                int short_max, int lane, TileStage &stage, TileResults &res) {
+    const TileMasks m = classify_edges<true, false>(p, q, kl, short_max, t, lane, nullptr, nullptr);
+    Pair a{__int_as_float(0x7FC00000), -1};
+    if ((m.longs >> lane) & 1) a = answer_edges<false, true>(e, kl, m.l, m.r);
+    if (m.shorts) {
+        answer_class<G, false, true, true>(kl, m.shorts, m, lane, 0, nullptr, nullptr, stage, &res);
+        if ((m.shorts >> lane) & 1) a = Pair{res.v[lane], res.i[lane]};
+    }
+    return a;
+}
+
+// The edge-table pass with the block sparse table copied into LDS first, for
+// a table that fits: the superset test and the edge blocks' (min, argmin)
+// then read LDS, and the vector-memory path (whose address and L1 stages
+// bound the global-table kernel: two divergent table reads per query)
+// carries only the queries, the results, edge pairs and scans.  Workgroups of
+// kLdsWaves waves, two resident per CU.  With its reads mostly from LDS the
+// pass streams queries and results, so it is pipelined on memory latency:
+// each wave keeps the queries of its next two tiles in flight, and every
+// full tile ends in exactly one store per output (OUT: 1 values, 2 indexes,
+// 3 both; scanned ranges' answers go through LDS to their own lanes), so the
+// compiler's wait before a tile's queries are used leaves the later loads and
+// stores outstanding.  The partial last tile is answered after the loop.
+template <int G, int OUT>
 __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, EdgeTables e, int table_pairs,
                                                                      const int2 *__restrict__ Q, int64_t q,
                                                                      int short_max, float *__restrict__ out_v,
                                                                      int32_t *__restrict__ out_i) {
     extern __shared__ Pair lds_table[];
     __shared__ TileStage stage[kLdsWaves];
+    __shared__ TileResults res[kLdsWaves];
     for (int j = threadIdx.x; j < table_pairs; j += kLdsWaves * kWave) lds_table[j] = k.st[j];
     __syncthreads();
     Blocks kl = k;
     kl.st = lds_table;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int64_t n_tiles = (q + kWave - 1) / kWave;
-    const int64_t wave_stride = (int64_t)gridDim.x * kLdsWaves;
-    // The next tile's queries are loaded while this tile is answered: with the
-    // table in LDS, the wait on them is most of a tile's latency.
+    const int64_t full = q / kWave;
+    const int64_t stride = (int64_t)gridDim.x * kLdsWaves;
     int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv;
-    int2 cur = fetch_query(Q, q, t, lane);
-    for (; t < n_tiles; t += wave_stride) {
-        const int2 nxt = fetch_query(Q, q, t + wave_stride, lane);
-        const TileMasks m = classify_edges(cur, q, kl, short_max, t, lane, out_v, out_i);
-        cur = nxt;
-        if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges<false, true>(e, kl, m.l, m.r);
-            write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
+    if (t < full) {
+        // tiles t, t+stride, t+2*stride... ; loads past the last full tile
+        // re-read that tile (always in bounds), so every iteration issues
+        // exactly one load and its count never depends on a branch
+        const int64_t last = full - 1;
+        int2 c0 = load_query(Q, t * kWave + lane);
+        int2 c1 = load_query(Q, min(t + stride, last) * kWave + lane);
+        for (; t < full; t += stride) {
+            const int2 c2 = load_query(Q, min(t + 2 * stride, last) * kWave + lane);
+            const Pair a = answer_tile_lds<G>(kl, e, c0, q, t, short_max, lane, stage[wv], res[wv]);
+            if constexpr (OUT & 1) __builtin_nontemporal_store(a.v, out_v + t * kWave + lane);
+            if constexpr (OUT & 2) __builtin_nontemporal_store(a.i, out_i + t * kWave + lane);
+            c0 = c1;
+            c1 = c2;
         }
-        if (m.shorts) answer_class<G, false>(kl, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
+    }
+    if (t == full && (q & (kWave - 1))) {  // the partial last tile falls to exactly this wave
+        const Pair a = answer_tile_lds<G>(kl, e, fetch_query(Q, q, t, lane), q, t, short_max, lane, stage[wv],
+                                          res[wv]);
+        if (t * kWave + lane < q) {
+            if constexpr (OUT & 1) __builtin_nontemporal_store(a.v, out_v + t * kWave + lane);
+            if constexpr (OUT & 2) __builtin_nontemporal_store(a.i, out_i + t * kWave + lane);
+        }
     }
 }
 
@@ -472,8 +508,19 @@ bool two_stage(const rmq_handle *h) { return h->st_bytes > kTwoStageTableBytes; 
 
 LongPass long_pass(const rmq_handle *h) {
     if (!h->edges.pre_b) return LongPass::kTiles;
-    return lds_table_fits(lds_table_bytes(h) + sizeof(TileStage) * kLdsWaves) ? LongPass::kEdgesLds
+    return lds_table_fits(lds_table_bytes(h) + (sizeof(TileStage) + sizeof(TileResults)) * kLdsWaves) ? LongPass::kEdgesLds
                                                                                 : LongPass::kEdges;
+}
+
+template <int G>
+void launch_edges_lds(int out, int grid, size_t lds, hipStream_t s, const Blocks &k, const EdgeTables &e, int pairs,
+                      const int2 *Q, int64_t q, int sm, float *val, int32_t *idx) {
+    const dim3 wg(kLdsWaves * kWave);
+    switch (out) {
+    case 1: k_query_edges_lds<G, 1><<<grid, wg, lds, s>>>(k, e, pairs, Q, q, sm, val, idx); break;
+    case 2: k_query_edges_lds<G, 2><<<grid, wg, lds, s>>>(k, e, pairs, Q, q, sm, val, idx); break;
+    default: k_query_edges_lds<G, 3><<<grid, wg, lds, s>>>(k, e, pairs, Q, q, sm, val, idx); break;
+    }
 }
 
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
@@ -490,11 +537,12 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
                                                                  (int64_t)device_cus() * 2));
             if (o.long_wgs > 0) g2 = std::min(g2, o.long_wgs);
             const int p = (int)(lds / sizeof(Pair));
+            const int out = (val ? 1 : 0) | (idx ? 2 : 0);
             switch (o.group_lanes) {
-            case 4: k_query_edges_lds<4><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
-            case 8: k_query_edges_lds<8><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
-            case 16: k_query_edges_lds<16><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
-            default: k_query_edges_lds<64><<<g2, kLdsWaves * kWave, lds, s>>>(k, h->edges, p, Q, q, sm, val, idx); break;
+            case 4: launch_edges_lds<4>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
+            case 8: launch_edges_lds<8>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
+            case 16: launch_edges_lds<16>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
+            default: launch_edges_lds<64>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
             }
             return hip_rc(hipGetLastError());
         }
@@ -780,7 +828,7 @@ RMQ_API int rmq_query_kernels(const rmq_handle_t *h, const rmq_query_opts_t *opt
     if ((rc = check_device(h)) != RMQ_OK) return rc;
     int w = 0;
     switch (long_pass(h)) {
-    case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d>", o.group_lanes); break;
+    case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d, 3>", o.group_lanes); break;
     case LongPass::kEdges:
         w = snprintf(buf, (size_t)len, "k_query_edges<%d, %s>", o.group_lanes, two_stage(h) ? "true" : "false");
         break;

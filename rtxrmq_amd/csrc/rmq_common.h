@@ -258,14 +258,14 @@ __device__ __forceinline__ int2 fetch_query(const int2 *__restrict__ Q, int64_t 
 }
 
 // load_tile_edges on a query already loaded (p = fetch_query(Q, q, t, lane)).
-template <bool NT = true>
+template <bool NT = true, bool WRITE_INVALID = true>
 __device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blocks &k, int short_max, int64_t t,
                                                     int lane, float *out_v, int32_t *out_i) {
     const int64_t qi = t * kWave + lane;
     const bool in = qi < q;
     const int l = p.x, r = p.y;
     const bool ok = in && l >= 0 && l <= r && r < k.n;
-    if (in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
+    if (WRITE_INVALID && in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     bool is_short = ok && (r - l) < short_max;
     if (ok && !is_short && r - l < kSub) {
         const int bl = block_of(l, k.magic);
@@ -527,15 +527,24 @@ struct TileStage {
     int lane[kWave];
 };
 
+// Per-wave LDS slots for one tile's scanned ranges: the lane group that
+// answers a range writes its (value, index) at the range's own lane, which
+// reads it back, so that every lane ends the tile with its answer in
+// registers and the tile's results leave in one store per output.
+struct TileResults {
+    float v[kWave];
+    int i[kWave];
+};
+
 // Answers the queries of one class in a tile (mask mk over the tile's lanes),
 // G lanes per query; results go to out[qbase + lane].  G == 64: the whole
 // wave per query, queries taken from the mask in lane order.  G < 64: the
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
-template <int G, bool LONG, bool NT = true>
+template <int G, bool LONG, bool NT = true, bool TO_LDS = false>
 __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
-                                             TileStage &stage) {
+                                             TileStage &stage, TileResults *res = nullptr) {
     constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
     if constexpr (LONG) {
         // One lane per query first: a range across blocks whose blocks'
@@ -566,7 +575,12 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             take_if_better(mv, mi, bv, bi);
             if (lane == 0) {
                 if (bv == INFINITY) bi = l;
-                write_result<NT>(out_v, out_i, qbase + src, bv, bi);
+                if constexpr (TO_LDS) {
+                    res->v[src] = bv;
+                    res->i[src] = bi;
+                } else {
+                    write_result<NT>(out_v, out_i, qbase + src, bv, bi);
+                }
             }
         }
     } else {
@@ -596,7 +610,12 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             take_if_better(mv, mi, bv, bi);
             if (has && gl == 0) {
                 if (bv == INFINITY) bi = ql;
-                write_result<NT>(out_v, out_i, qbase + stage.lane[s], bv, bi);
+                if constexpr (TO_LDS) {
+                    res->v[stage.lane[s]] = bv;
+                    res->i[stage.lane[s]] = bi;
+                } else {
+                    write_result<NT>(out_v, out_i, qbase + stage.lane[s], bv, bi);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();

@@ -135,7 +135,7 @@ def test_query_kernel_names(gpu):
     A = _dev(np.random.default_rng(3).random(1 << 20).astype(np.float32), gpu)
     opts = R.query_opts(short_max=32, group_lanes=8, long_lanes=16)
     with R.RMQIndex(A, 1 << 15, flags=R.BUILD_EDGES) as ix:      # 32 blocks: a few KB
-        assert ix.kernels(opts) == ["k_query_edges_lds<8>"]
+        assert ix.kernels(opts) == ["k_query_edges_lds<8, 3>"]
     with R.RMQIndex(A, 64, flags=R.BUILD_EDGES) as ix:           # 16384 blocks x 15 levels: about 2 MB
         assert ix.kernels(opts) == ["k_query_edges<8, false>"]
     big = _dev(np.random.default_rng(4).random(1 << 24).astype(np.float32), gpu)
@@ -260,7 +260,7 @@ def test_headline_full_size(lr, flags, gpu):
 
 
 # The kernels the committed MI355X picks run for the three headline batches.
-HEADLINE_KERNELS = {-1: ["k_query_edges_lds<16>"], -2: ["k_query_edges<4, false>"], -3: ["k_query_edges<8, true>"]}
+HEADLINE_KERNELS = {-1: ["k_query_edges_lds<16, 3>"], -2: ["k_query_edges<4, false>"], -3: ["k_query_edges<8, true>"]}
 
 
 @pytest.mark.parametrize("lr", [-1, -2, -3])
