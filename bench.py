@@ -970,8 +970,14 @@ def run_served(R, torch, dev, args) -> dict:
     poi, poi_rate, poi_bad = run_stream(args.serve_requests, offered, f"poisson {offered:.0f} req/s")
     sv.close()
     staged = run_staged(R, args, arrays, bs, flags, opts, payloads, plain_out, rng)
+    # pinned host blocks go back to the runtime now, while it is whole, rather
+    # than from torch's allocator during library teardown at exit
+    pq_count = len(payloads)
+    del payloads, arena, out_pin, plain_out
+    torch.cuda.synchronize()
+    torch._C._host_emptyCache()
     return {"arrays": m, "elements": int(ns.sum()), "bs": bs, "edge_tables": bool(flags), "slots": args.serve_slots,
-            "request_q": "log-uniform [2^8, 2^22]", "mean_q": float(pq.mean()), "payloads": len(payloads),
+            "request_q": "log-uniform [2^8, 2^22]", "mean_q": float(pq.mean()), "payloads": pq_count,
             "plain_rmq_per_s": plain_rate, "plain_requests": len(order),
             "saturation": {"rmq_per_s": sat_rate, "requests": sat["requests_done"], "p50_ms": sat["p50_ms"],
                            "p99_ms": sat["p99_ms"], "requests_per_launch": sat["mean_requests_per_launch"]},
