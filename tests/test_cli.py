@@ -166,6 +166,47 @@ def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
     assert m and float(m.group(1)) > 0
 
 
+def test_save_power_hrmq_alg(tmp_path):
+    """--save-power for alg 1 (HRMQ): the package energy of a fake powercap
+    tree sampled into the reference's six columns, and RMQ/J printed, as the
+    reference reads RAPL around rmq_rmm_par (src/cpu_methods.h:100-102)."""
+    import threading
+    import time
+
+    root = tmp_path / "powercap"
+    pkg = root / "intel-rapl:0"
+    pkg.mkdir(parents=True)
+    (pkg / "name").write_text("package-0\n")
+    (pkg / "energy_uj").write_text("0\n")
+    (pkg / "max_energy_range_uj").write_text("1000000000\n")
+    stop = threading.Event()
+
+    def tick():
+        e = 0
+        while not stop.is_set():
+            time.sleep(0.05)
+            e += 500_000
+            tmp = pkg / "energy_uj.tmp"
+            tmp.write_text(f"{e}\n")
+            os.replace(tmp, pkg / "energy_uj")
+
+    th = threading.Thread(target=tick)
+    th.start()
+    pfile = tmp_path / "hrmq.dat"
+    try:
+        env = dict(os.environ, RTXRMQ_POWERCAP=str(root))
+        r = subprocess.run([CLI, "1000000", "1000000", "-1", "1", "--nt", "1", "--reps", "12", "--seed", "3",
+                            f"--save-power={pfile}"], capture_output=True, text=True, timeout=300, env=env)
+    finally:
+        stop.set()
+        th.join()
+    out = ANSI.sub("", r.stdout)
+    assert r.returncode == 0, r.stderr[-1500:]
+    rows = [ln.split() for ln in pfile.read_text().splitlines() if ln.strip() and not ln.startswith("#")]
+    assert len(rows) >= 1 and all(len(x) == 6 for x in rows)
+    assert "RMQ/J" in out
+
+
 def test_save_power_cpu_alg_reads_powercap_packages(tmp_path):
     """CPU algs sample the CPU packages' energy every 500 ms, as the reference
     samples RAPL around its CPU methods (common/nvmlPower.cpp:216-274), into

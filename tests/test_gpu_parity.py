@@ -337,6 +337,45 @@ def test_invalid_queries_are_flagged_not_faulted(gpu):
     assert R.check_queries(_dev(Q, gpu), 1000) == 3
 
 
+@pytest.mark.parametrize("outputs", ["vi", "v", "i"])
+@pytest.mark.parametrize("bs,flags,family", [(1024, R.BUILD_EDGES, "k_query_edges_lds<"),
+                                             (8, R.BUILD_EDGES, "k_query_edges<"), (4096, 0, "k_query_tiles<")])
+def test_invalid_and_partial_tiles_every_kernel_and_output(bs, flags, family, outputs, gpu):
+    """Each query kernel (LDS-table, global-table, tile pair) with each output
+    kind: batches of 1, 63, 64, 65, 1000 and 4097 queries (full and partial
+    64-query tiles, fewer tiles than waves), a tenth of them invalid (l > r,
+    l < 0, r >= n), mixed with short and long ranges; valid answers equal
+    brute force, invalid ones are (NaN, -1), and nothing past q is written."""
+    n = 100_003
+    A = O.gen_array(n, 17, "ties16")
+    A_dev = _dev(A, gpu)
+    rng = np.random.default_rng(bs + len(outputs))
+    with R.RMQIndex(A_dev, bs, flags=flags) as ix:
+        assert ix.kernels()[0].startswith(family), ix.kernels()
+        for q in (1, 63, 64, 65, 1000, 4097):
+            l = rng.integers(0, n, q)
+            ln = np.where(rng.random(q) < 0.5, rng.integers(1, 40, q), rng.integers(1, 5000, q))
+            Q = np.stack([l, np.minimum(n - 1, l + ln - 1)], 1).astype(np.int32)
+            bad = rng.random(q) < 0.1
+            kinds = rng.integers(0, 3, q)
+            Q[bad & (kinds == 0)] = [[7, 3]]
+            Q[bad & (kinds == 1)] = [[-2, 5]]
+            Q[bad & (kinds == 2)] = [[n - 5, n + 2]]
+            ev, ei = O.rmq_brute(A, np.where(bad[:, None], 0, Q))
+            v = torch.full((q + 8,), 123.0, dtype=torch.float32, device=gpu)
+            i = torch.full((q + 8,), 77, dtype=torch.int32, device=gpu)
+            ix.query(_dev(Q, gpu), want_value="v" in outputs, want_index="i" in outputs,
+                     out_val=v if "v" in outputs else None, out_idx=i if "i" in outputs else None)
+            torch.cuda.synchronize()
+            vh, ih = v.cpu().numpy(), i.cpu().numpy()
+            if "i" in outputs:
+                assert np.array_equal(ih[:q][~bad], ei[~bad]) and np.all(ih[:q][bad] == -1), f"q={q}"
+                assert np.all(ih[q:] == 77), f"q={q}: index written past q"
+            if "v" in outputs:
+                assert np.array_equal(vh[:q][~bad].view(np.int32), ev[~bad].view(np.int32)), f"q={q}"
+                assert np.all(np.isnan(vh[:q][bad])) and np.all(vh[q:] == 123.0), f"q={q}"
+
+
 @pytest.mark.parametrize("flags", FLAGS)
 def test_empty_and_single(flags, gpu):
     A = _dev(np.array([2.5], np.float32), gpu)
