@@ -267,10 +267,10 @@ __device__ __forceinline__ Pair answer_tile_lds(const Blocks &kl, const EdgeTabl
                                                 int short_max, int lane, TileStage &stage, TileResults &res) {
     const TileMasks m = classify_edges<true, false>(p, q, kl, short_max, t, lane, nullptr, nullptr);
     Pair a{__int_as_float(0x7FC00000), -1};
-    if ((m.longs >> lane) & 1) a = answer_edges<false, true>(e, kl, m.l, m.r);
+    if (m.own_long) a = answer_edges<false, true>(e, kl, m.l, m.r);
     if (m.shorts) {
         answer_class<G, false, true, true>(kl, m.shorts, m, lane, 0, nullptr, nullptr, stage, &res);
-        if ((m.shorts >> lane) & 1) a = Pair{res.v[lane], res.i[lane]};
+        if (m.own_short) a = Pair{res.v[lane], res.i[lane]};
     }
     return a;
 }
@@ -286,7 +286,11 @@ __device__ __forceinline__ Pair answer_tile_lds(const Blocks &kl, const EdgeTabl
 // full tile ends in exactly one store per output (OUT: 1 values, 2 indexes,
 // 3 both; scanned ranges' answers go through LDS to their own lanes), so the
 // compiler's wait before a tile's queries are used leaves the later loads and
-// stores outstanding.  The partial last tile is answered after the loop.
+// stores outstanding.  The loop is unrolled three times with the three query
+// registers in rotation: written as a copy (c0 = c1; c1 = c2) at the back
+// edge, the copy of a register whose load is in flight waits for that load,
+// which left only one tile's work to cover each load.  The partial last tile
+// is answered after the loop.
 template <int G, int OUT>
 __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, EdgeTables e, int table_pairs,
                                                                      const int2 *__restrict__ Q, int64_t q,
@@ -306,19 +310,40 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
     int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv;
     if (t < full) {
         // tiles t, t+stride, t+2*stride... ; loads past the last full tile
-        // re-read that tile (always in bounds), so every iteration issues
-        // exactly one load and its count never depends on a branch
+        // re-read that tile (always in bounds), so every step issues exactly
+        // one load and its count never depends on a branch
         const int64_t last = full - 1;
-        int2 c0 = load_query(Q, t * kWave + lane);
-        int2 c1 = load_query(Q, min(t + stride, last) * kWave + lane);
-        for (; t < full; t += stride) {
-            const int2 c2 = load_query(Q, min(t + 2 * stride, last) * kWave + lane);
-            const Pair a = answer_tile_lds<G>(kl, e, c0, q, t, short_max, lane, stage[wv], res[wv]);
-            if constexpr (OUT & 1) __builtin_nontemporal_store(a.v, out_v + t * kWave + lane);
-            if constexpr (OUT & 2) __builtin_nontemporal_store(a.i, out_i + t * kWave + lane);
-            c0 = c1;
-            c1 = c2;
-        }
+        auto ahead = [&](int64_t u) { return load_query(Q, min(u, last) * kWave + lane); };
+        auto tile = [&](int2 c, int64_t u) {
+            const Pair a = answer_tile_lds<G>(kl, e, c, q, u, short_max, lane, stage[wv], res[wv]);
+            if constexpr (OUT & 1) __builtin_nontemporal_store(a.v, out_v + u * kWave + lane);
+            if constexpr (OUT & 2) __builtin_nontemporal_store(a.i, out_i + u * kWave + lane);
+        };
+        int2 c0 = ahead(t), c1 = ahead(t + stride), c2;
+        do {
+            // two steps ahead of the loop, so that it is entered, as at its
+            // back edge, with a full step's stores and load behind each
+            // query register: the wait before a register's first use is
+            // then the same on both paths into the loop and keeps the next
+            // two tiles' loads in flight
+            c2 = ahead(t + 2 * stride);
+            tile(c0, t);
+            if ((t += stride) >= full) break;
+            c0 = ahead(t + 2 * stride);
+            tile(c1, t);
+            if ((t += stride) >= full) break;
+            for (;;) {
+                c1 = ahead(t + 2 * stride);
+                tile(c2, t);
+                if ((t += stride) >= full) break;
+                c2 = ahead(t + 2 * stride);
+                tile(c0, t);
+                if ((t += stride) >= full) break;
+                c0 = ahead(t + 2 * stride);
+                tile(c1, t);
+                if ((t += stride) >= full) break;
+            }
+        } while (false);
     }
     if (t == full && (q & (kWave - 1))) {  // the partial last tile falls to exactly this wave
         const Pair a = answer_tile_lds<G>(kl, e, fetch_query(Q, q, t, lane), q, t, short_max, lane, stage[wv],

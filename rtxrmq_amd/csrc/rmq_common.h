@@ -198,6 +198,8 @@ struct TileMasks {
     int l, r;          // this lane's query
     uint64_t shorts;   // lanes holding a valid range of at most short_max elements
     uint64_t longs;    // lanes holding a valid longer range
+    bool own_short;    // this lane's bit of shorts, without a 64-bit shift and test
+    bool own_long;     // this lane's bit of longs
 };
 
 template <bool NT = true>
@@ -214,16 +216,18 @@ __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64
     const bool ok = in && l >= 0 && l <= r && r < n;
     if (flag_invalid && in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     const bool is_short = ok && (r - l) < short_max;
-    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short), is_short, ok && !is_short};
 }
 
 // Block index of element e: e / bs by one multiply-high with the handle's
 // magic = floor((2^64-1)/bs) + 1 (exact for 0 <= e < 2^31 and bs >= 2);
-// magic 0 means bs == 1.
+// magic 0 means bs == 1.  Every caller passes an element index (e >= 0), so
+// e is zero-extended: a 32 x 64-bit product, two VALU multiplies where the
+// sign-extended form takes about nine.
 __host__ __device__ __forceinline__ uint64_t block_magic(int bs) { return bs > 1 ? UINT64_MAX / (uint64_t)bs + 1 : 0; }
 
 __device__ __forceinline__ int block_of(int e, uint64_t magic) {
-    return magic ? (int)__umul64hi((uint64_t)e, magic) : e;
+    return magic ? (int)__umul64hi((uint64_t)(uint32_t)e, magic) : e;
 }
 
 // Leftmost minimum of the c >= 1 entries [a, a+c) of a sparse table whose
@@ -272,7 +276,7 @@ __device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blo
         const int base = bl * k.bs;  // <= l
         is_short = block_of(r, k.magic) == bl && ((l - base) >> kSubShift) == ((r - base) >> kSubShift);
     }
-    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short)};
+    return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short), is_short, ok && !is_short};
 }
 
 template <bool NT = true>
