@@ -299,7 +299,20 @@ __global__ __launch_bounds__(kLdsWaves *kWave) void k_query_edges_lds(Blocks k, 
     extern __shared__ Pair lds_table[];
     __shared__ TileStage stage[kLdsWaves];
     __shared__ TileResults res[kLdsWaves];
-    for (int j = threadIdx.x; j < table_pairs; j += kLdsWaves * kWave) lds_table[j] = k.st[j];
+    {
+        // four loads in flight per thread, then their LDS writes: one memory
+        // round trip per 4 x 1024 pairs rather than per 1024
+        constexpr int kThreads = kLdsWaves * kWave, kCopyU = 4;
+        for (int j0 = threadIdx.x; j0 < table_pairs; j0 += kCopyU * kThreads) {
+            Pair x[kCopyU];
+#pragma unroll
+            for (int u = 0; u < kCopyU; ++u)
+                if (j0 + u * kThreads < table_pairs) x[u] = k.st[j0 + u * kThreads];
+#pragma unroll
+            for (int u = 0; u < kCopyU; ++u)
+                if (j0 + u * kThreads < table_pairs) lds_table[j0 + u * kThreads] = x[u];
+        }
+    }
     __syncthreads();
     Blocks kl = k;
     kl.st = lds_table;
