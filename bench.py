@@ -424,10 +424,16 @@ def reduce_pmc(d: str, copy_bytes: int) -> int:
 
     sections = [(line["roofline"]["workload"], line["per_kernel"])]
     sections += [(v["roofline"]["workload"], v["per_kernel"]) for k, v in line.items() if k.startswith("fixed_bs_")]
-    out = {}  # only the workloads this bench line ran: figures for older configurations are dropped
+    # only the workloads this bench line ran, and of each only the kernels it launches now: figures
+    # for older configurations and for kernels since renamed or replaced are dropped
+    current = {}
+    for wk, kernels in sections:
+        current.setdefault(wk, set()).update(kernels)
+    out = {}
     if os.path.exists(TRAFFIC_PATH):
         with open(TRAFFIC_PATH) as f:
-            out = {w: v for w, v in json.load(f).items() if w in {wk for wk, _ in sections}}
+            out = {w: {k: r for k, r in v.items() if k in current[w]}
+                   for w, v in json.load(f).items() if w in current}
     seen = [sym for _, kernels in sections for sym in kernels]
     for workload, kernels in sections:
         for sym, info in kernels.items():
