@@ -223,7 +223,11 @@ int rmq_query_loop(const rmq_handle_t *const *handles, int32_t m, const int32_t 
  *      query requests over arrays kept resident in HBM.
  *      - rmq_server_add_array builds a handle for a caller-owned device array
  *        (kept alive by the caller) after checking that its block table fits
- *        in free HBM beside the request budget; returns its id.
+ *        in free HBM beside the request budget; returns its id.  The build
+ *        runs on the server's own stream, unordered with the caller's: the
+ *        array's values must be complete when the call is made.  Launches
+ *        and completions of requests go on while it builds; adds run one at
+ *        a time.
  *      - rmq_server_submit queues a request: q host query pairs for one array
  *        and host output buffers; it returns at once with a ticket.  The
  *        buffers must stay valid, and the queries unchanged, until

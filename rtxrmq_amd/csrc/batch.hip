@@ -497,6 +497,8 @@ struct rmq_server {
     int32_t descs_cap = 0;
     PlanKernel kernel = nullptr;
     std::vector<Slot> slots;
+    hipStream_t build_stream = nullptr;  // builds of added arrays, apart from the slots' launches
+    std::mutex add_mu;  // one rmq_server_add_array at a time; held without mu while a table builds
     std::mutex mu;
     std::condition_variable cv_sched, cv_done, cv_slot;
     std::deque<uint64_t> pending;
@@ -744,6 +746,10 @@ void server_release(rmq_server *sv) {
         if (s.d_items) (void)hipFree(s.d_items);
         if (s.d_head) (void)hipFree(s.d_head);
     }
+    if (sv->build_stream) {
+        (void)hipStreamSynchronize(sv->build_stream);
+        (void)hipStreamDestroy(sv->build_stream);
+    }
     for (rmq_handle *h : sv->arrays) rmq_free(h);
     if (sv->d_descs) (void)hipFree(sv->d_descs);
     if (sv->pool) (void)hipMemPoolDestroy(sv->pool);  // every staging buffer was freed on a synchronised stream
@@ -796,6 +802,7 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
         rc = hip_rc(hipMalloc(&sv->d_descs, sizeof(ArrayDesc) * (size_t)c.max_arrays));
         sv->descs_cap = c.max_arrays;
     }
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamCreateWithFlags(&sv->build_stream, hipStreamNonBlocking));
     sv->slots.resize((size_t)c.slots);
     for (Slot &s : sv->slots) {
         if (rc == RMQ_OK) rc = hip_rc(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
@@ -820,37 +827,43 @@ RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t 
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;
     if (dev != sv->device) return RMQ_ERR_DEVICE;
-    std::lock_guard<std::mutex> lk(sv->mu);
-    if ((int32_t)sv->arrays.size() >= sv->descs_cap) return RMQ_ERR_OOM;
-    // admission of a new resident array: its tables (exactly what rmq_build
-    // allocates) must fit in free HBM beside the request budget still available
+    // Adds are serialised by add_mu; the server lock is held only to admit
+    // and to publish, so launches and completions go on while a table builds.
+    std::lock_guard<std::mutex> add_lk(sv->add_mu);
     const int64_t table = handle_sizes(n, bs, flags).total();
-    size_t free_b = 0, total_b = 0;
-    int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
-    if (rc != RMQ_OK) return rc;
-    // A configured budget is a promise: its unspent part stays free.  A budget
-    // derived from free HBM yields to resident arrays instead (it shrinks
-    // below, once the tables are built); only the runtime reserve stays free.
-    const int64_t promised = sv->budget_fixed ? sv->budget - sv->inflight : sv->reserve;
-    if (table + promised > (int64_t)free_b) return RMQ_ERR_OOM;
+    int32_t id = 0;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if ((int32_t)sv->arrays.size() >= sv->descs_cap) return RMQ_ERR_OOM;
+        // admission of a new resident array: its tables (exactly what
+        // rmq_build allocates) must fit in free HBM beside the request budget
+        // still available.  A configured budget is a promise: its unspent part
+        // stays free (requests launched during the build spend from that part
+        // only).  A budget derived from free HBM yields to resident arrays
+        // instead (it shrinks below, once the tables are built); only the
+        // runtime reserve stays free.
+        size_t free_b = 0, total_b = 0;
+        const int rc = hip_rc(hipMemGetInfo(&free_b, &total_b));
+        if (rc != RMQ_OK) return rc;
+        const int64_t promised = sv->budget_fixed ? sv->budget - sv->inflight : sv->reserve;
+        if (table + promised > (int64_t)free_b) return RMQ_ERR_OOM;
+        id = (int32_t)sv->arrays.size();  // stays the next id: only an add (under add_mu) appends
+    }
     rmq_handle_t *h = nullptr;
-    hipStream_t s = sv->slots[0].stream;
-    rc = rmq_build(d_A, n, bs, flags, s, &h);
-    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(s));
+    int rc = rmq_build(d_A, n, bs, flags, sv->build_stream, &h);
+    if (rc == RMQ_OK) rc = hip_rc(hipStreamSynchronize(sv->build_stream));
+    // the entry is written beyond every entry an in-flight launch can read:
+    // requests name it only once it is published below
+    const ArrayDesc d = rc == RMQ_OK ? desc_of(h) : ArrayDesc{};
+    if (rc == RMQ_OK) rc = hip_rc(hipMemcpy(sv->d_descs + id, &d, sizeof(d), hipMemcpyHostToDevice));
     if (rc != RMQ_OK) {
         rmq_free(h);
         return rc;
     }
-    const ArrayDesc d = desc_of(h);
-    const int32_t id = (int32_t)sv->arrays.size();
-    // a new entry is written beyond every entry an in-flight launch can read
-    rc = hip_rc(hipMemcpy(sv->d_descs + id, &d, sizeof(d), hipMemcpyHostToDevice));
-    if (rc != RMQ_OK) {
-        rmq_free(h);
-        return rc;
-    }
+    std::lock_guard<std::mutex> lk(sv->mu);
     sv->arrays.push_back(h);
     *array_id = id;
+    size_t free_b = 0, total_b = 0;
     if (!sv->budget_fixed && hipMemGetInfo(&free_b, &total_b) == hipSuccess)
         sv->budget = std::max<int64_t>(0, std::min<int64_t>(sv->budget, sv->inflight + (int64_t)free_b - sv->reserve));
     return RMQ_OK;

@@ -54,11 +54,15 @@ def _dev_f32(t, name: str):
     return t
 
 
-def _dev_queries(t):
+def _dev_queries(t, device=None):
+    """Queries for a kernel: a contiguous int32 CUDA tensor of {l, r} pairs,
+    on `device` (the array's) when one is given -- the kernels read it there."""
     torch = _torch()
     if (not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.int32
             or not t.is_contiguous() or t.numel() % 2):
         raise TypeError("queries must be a contiguous int32 CUDA tensor of {l, r} pairs")
+    if device is not None and t.device != device:
+        raise TypeError(f"queries are on {t.device}, the array on {device}")
     return t
 
 
@@ -164,7 +168,7 @@ class RMQIndex:
 
     def query(self, Q, want_value: bool = True, want_index: bool = True, opts: N.QueryOpts | None = None,
               out_val=None, out_idx=None, stream=None):
-        Q = _dev_queries(Q)
+        Q = _dev_queries(Q, self.A.device)
         q = Q.numel() // 2
         v, i = self._outs(q, want_value, want_index, out_val, out_idx)
         rc = N.lib().rmq_query_ex(self._h, Q.data_ptr(), q, v.data_ptr() if v is not None else None,
@@ -175,7 +179,7 @@ class RMQIndex:
 
     def query_timed(self, Q, want_value: bool = True, want_index: bool = True, opts: N.QueryOpts | None = None,
                     out_val=None, out_idx=None, stream=None):
-        Q = _dev_queries(Q)
+        Q = _dev_queries(Q, self.A.device)
         q = Q.numel() // 2
         v, i = self._outs(q, want_value, want_index, out_val, out_idx)
         t = N.QueryTimes()
@@ -212,7 +216,10 @@ class HostHRMQ:
     its bit patterns, src/main.cu:117-125).  Runs on the CPU only."""
 
     def __init__(self, A: np.ndarray):
-        self.A = np.ascontiguousarray(A).view(np.int32)
+        A = np.ascontiguousarray(A)
+        if A.dtype not in (np.dtype(np.float32), np.dtype(np.int32)) or A.ndim != 1:
+            raise TypeError("HostHRMQ takes a 1-d float32 or int32 array")
+        self.A = A.view(np.int32)
         h = ctypes.c_void_p()
         N.check(N.lib().rmq_hrmq_build(self.A.ctypes.data, self.A.size, ctypes.byref(h)), "rmq_hrmq_build")
         self._h = h
@@ -264,7 +271,7 @@ def gpu_base(A, Q, want_value: bool = True, want_index: bool = True, stream=None
     """Brute force on the device (algs 2/102; the --check oracle)."""
     torch = _torch()
     A = _dev_f32(A, "A")
-    Q = _dev_queries(Q)
+    Q = _dev_queries(Q, A.device)
     q = Q.numel() // 2
     v = torch.empty(q, dtype=torch.float32, device=A.device) if want_value else None
     i = torch.empty(q, dtype=torch.int32, device=A.device) if want_index else None
@@ -296,7 +303,7 @@ def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool
     """
     torch = _torch()
     A = _dev_f32(A, "A")
-    Q_sample = _dev_queries(Q_sample)
+    Q_sample = _dev_queries(Q_sample, A.device)
     if not (want_value or want_index):
         raise ValueError("tune: time at least one output")
     q = Q_sample.numel() // 2
@@ -377,7 +384,7 @@ class Plan:
         m = len(indexes)
         if len(Qs) != m or (out_idx is not None and len(out_idx) != m) or (out_val is not None and len(out_val) != m):
             raise ValueError("indexes, Qs and outputs must have the same length")
-        Qs = [_dev_queries(Q) for Q in Qs]
+        Qs = [_dev_queries(Q, ix.A.device) for ix, Q in zip(indexes, Qs)]
         for k, (ix, Q) in enumerate(zip(indexes, Qs)):
             q = Q.numel() // 2
             i_t = out_idx[k] if out_idx is not None else None
@@ -455,7 +462,12 @@ class Server:
         self._live = {}
 
     def add_array(self, A, bs: int, flags: int = 0) -> int:
+        """Builds A's tables on the server's own stream, which is not ordered
+        after the caller's: the current stream (which wrote A, as torch and
+        gen_array do) is synchronised first.  An array written on another
+        stream must be finished before this call."""
         A = _dev_f32(A, "A")
+        _torch().cuda.current_stream(A.device).synchronize()
         aid = ctypes.c_int32()
         N.check(N.lib().rmq_server_add_array_ex(self._h, A.data_ptr(), A.numel(), bs, flags, ctypes.byref(aid)),
                 "rmq_server_add_array")

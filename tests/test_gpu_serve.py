@@ -160,3 +160,39 @@ def test_pinned_buffers_reused_with_new_contents(arrays, stage, gpu):
         with pytest.raises(TypeError):
             sv.submit(aid, Qp, out_idx=np.empty(q - 1, np.int32))
     assert st["failed"] == 0 and st["peak_inflight_bytes"] > 0  # the staged request used device memory
+
+
+def test_arrays_added_while_requests_run(arrays, gpu):
+    """An array added while requests on others are in flight: the build runs
+    on the server's own stream outside its lock, and the ids, the descriptor
+    table and every answer stay right (arrays generated on the device just
+    before being added: the add must see their finished values)."""
+    A0, Ad0 = arrays[2]
+    with R.Server(slots=2) as sv:
+        a0 = sv.add_array(Ad0, 256, R.BUILD_EDGES)
+        first = [O.gen_queries(200_000, A0.size, lr, 70 + k) for k, lr in enumerate((-1, -7, -8, -1))]
+        first_out = [np.empty(Q.shape[0], np.int32) for Q in first]
+        tks = [sv.submit(a0, Q, out_idx=o) for Q, o in zip(first, first_out)]
+        added = []
+        for k, (n, kind) in enumerate([(300_001, "ties16"), (1 << 19, "uniform")]):
+            Ad = R.gen_array(n, 900 + k, kind, device=gpu)  # on torch's stream, not yet finished
+            added.append((sv.add_array(Ad, (64, 1024)[k], R.BUILD_EDGES if k else 0), Ad))
+        assert [aid for aid, _ in added] == [a0 + 1, a0 + 2]
+        second, second_out = [], []
+        for aid, Ad in added:
+            Q = O.gen_queries(100_000, Ad.numel(), -8, aid)
+            second.append(Q)
+            second_out.append(np.empty(Q.shape[0], np.int32))
+            tks.append(sv.submit(aid, Q, out_idx=second_out[-1]))
+        assert all(sv.wait(tk)[0] == R.RMQ_OK for tk in tks)
+        hosts = [Ad.cpu().numpy() for _, Ad in added]
+    f = O.FastOracle(A0)
+    for Q, o in zip(first, first_out):
+        np.testing.assert_array_equal(o, f.query(Q)[1])
+    f.close()
+    for k, (A, Q, o) in enumerate(zip(hosts, second, second_out)):
+        kind = ("ties16", "uniform")[k]
+        np.testing.assert_array_equal(A.view(np.int32), O.gen_array(A.size, 900 + k, kind).view(np.int32))
+        f = O.FastOracle(A)
+        np.testing.assert_array_equal(o, f.query(Q)[1], err_msg=f"added array {k}")
+        f.close()
