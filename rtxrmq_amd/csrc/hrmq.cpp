@@ -29,6 +29,7 @@
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <thread>
 #include <vector>
@@ -222,10 +223,8 @@ struct rmq_hrmq {
         A = a;
         n = len;
         nbits = 2 * (n + 1);
-        const int64_t words = (nbits + 63) / 64;
         const int64_t supers = (nbits + kSuperBits - 1) / kSuperBits;
         P.assign((size_t)(supers * (kSuperBits / 64)), 0ULL);  // whole superblocks: rank reads stay in bounds
-        (void)words;
         // the 2d-min-heap's parentheses, with an explicit stack
         std::vector<int32_t> stack;
         stack.reserve(1024);
@@ -317,12 +316,22 @@ RMQ_API int rmq_hrmq_query(const rmq_hrmq_t *h, const int32_t *Q, int64_t q, int
         work(0, q);
         return RMQ_OK;
     }
-    // static chunks, as OpenMP's default schedule in the reference's loop
+    // static chunks, as OpenMP's default schedule in the reference's loop; a
+    // chunk whose thread cannot be started runs on the calling thread
     std::vector<std::thread> pool;
     const int64_t chunk = (q + nt - 1) / nt;
+    try {
+        pool.reserve((size_t)nt);
+    } catch (const std::exception &) {
+    }
     for (int t = 1; t < nt; ++t) {
         const int64_t lo = t * chunk, hi = std::min<int64_t>(q, lo + chunk);
-        if (lo < hi) pool.emplace_back(work, lo, hi);
+        if (lo >= hi) continue;
+        try {
+            pool.emplace_back(work, lo, hi);
+        } catch (const std::exception &) {
+            work(lo, hi);
+        }
     }
     work(0, std::min<int64_t>(q, chunk));
     for (auto &th : pool) th.join();

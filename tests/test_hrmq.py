@@ -96,6 +96,9 @@ def test_invalid_queries_and_arguments():
     assert lib.rmq_hrmq_build(None, 10, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_hrmq_build(A.ctypes.data, 0, ctypes.byref(h)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_hrmq_query(None, Q.ctypes.data, 4, None, None, 0) == R.RMQ_ERR_INVALID_ARG
+    for bad in (np.arange(10, dtype=np.float64), np.arange(10, dtype=np.int16), np.zeros((2, 5), np.float32)):
+        with pytest.raises(TypeError):
+            R.HostHRMQ(bad)
 
 
 @pytest.mark.skipif(not O.HRMQ.available(), reason="reference HRMQ not built (oracle/_ref)")
