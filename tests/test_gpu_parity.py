@@ -89,6 +89,30 @@ def test_long_scans_through_sub_blocks(kind, bs, flags, gpu):
     assert_exact(v, i, ev, ei, f"{kind} bs={bs} flags={flags}")
 
 
+@pytest.mark.parametrize("bs,flags", [(8, 0), (8, R.BUILD_EDGES), (37, R.BUILD_EDGES), (1000, 0),
+                                      (1000, R.BUILD_EDGES), (1 << 15, 0), (1 << 17, R.BUILD_EDGES)])
+def test_signed_zeros_infinities_and_denormals(bs, flags, gpu):
+    """-0.0 and +0.0 compare equal, so the leftmost of them is the answer and
+    its sign bit must come back exactly; -inf, +inf, denormals and negative
+    values mixed in heavy ties, through every query kernel (global sparse
+    table, table in LDS, tile scans) and every range class."""
+    n = 300_007
+    rng = np.random.default_rng(bs + flags)
+    pool = np.array([0.0, -0.0, np.inf, 1e-45, 1.4e-44, 2.0], np.float32)  # the minimum is mostly a zero tie
+    A = pool[rng.integers(0, pool.size, n)]
+    run = (np.arange(n) // 5000) % 4 == 1  # stretches of positive values only: a denormal minimum
+    A[run] = np.where(A[run] == 0, np.float32(1e-45), A[run])
+    for x, k in ((-1e-45, 30), (-3.5, 20), (-np.inf, 10)):  # rare negatives
+        A[rng.integers(0, n, k)] = np.float32(x)
+    Q = np.concatenate([O.gen_queries(30_000, n, lr, bs) for lr in (-1, -2, -3, 1, 33)])
+    ev, ei = O.rmq_brute(A, Q)
+    for opts in (None, R.query_opts(short_max=1, group_lanes=4, long_lanes=4)):
+        v, i = run_engine(A, Q, bs, gpu, opts=opts, flags=flags)
+        assert_exact(v, i, ev, ei, f"bs={bs} flags={flags} opts={opts is not None}")
+    bits = ev.view(np.uint32)
+    assert (bits == 0).any() and (bits == 0x80000000).any() and (bits == 1).any()  # +0, -0, 1e-45 all answered
+
+
 @pytest.fixture(scope="module")
 def mixed_case():
     A = O.gen_array(200003, 11, "ties16")
