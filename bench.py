@@ -521,7 +521,7 @@ def compare_hrmq(A_host: np.ndarray, val_g, idx_g, val_h, idx_h) -> dict:
 
 
 def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers: dict,
-                 config1: dict | None) -> dict | None:
+                 plain_small: dict | None) -> dict | None:
     """The reference's HRMQ on this host's cores over a bounded sample of each
     batch, checked against the GPU's answers to the same queries; this
     build's own HRMQ (algs 1/101 of its CLI) on the same sample beside it;
@@ -577,9 +577,9 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
     h.close()
     port.close()
     small = None
-    if config1:
+    if plain_small:
         # CPU BASE (brute force from A[l], strict '<'), bounded sample; HRMQ on every query
-        A1, Q1 = config1["A"], config1["Q"]
+        A1, Q1 = plain_small["A"], plain_small["Q"]
         pilot = Q1[:64]
         t0 = time.perf_counter()
         O.rmq_brute(A1, pilot, nt=nt)
@@ -588,7 +588,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
         t0 = time.perf_counter()
         bv, bi = O.rmq_brute(A1, Q1[:m], nt=nt)
         base_dt = time.perf_counter() - t0
-        g_v, g_i = config1["val"], config1["idx"]
+        g_v, g_i = plain_small["val"], plain_small["idx"]
         base_bad = int(((bi != g_i[:m]) | (bv.view(np.int32) != g_v[:m].view(np.int32))).sum())
         h1 = O.HRMQ(A1)
         t0 = time.perf_counter()
@@ -597,13 +597,13 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
         h1.close()
         cmp = compare_hrmq(A1, g_v, g_i, hv, hi)
         small = {"n": int(A1.size), "q": int(Q1.shape[0]), "lr": -1,
-                 "gpu_rmq_per_s": config1["rmq_per_s"], "gpu_bs": config1["bs"],
+                 "gpu_rmq_per_s": plain_small["rmq_per_s"], "gpu_bs": plain_small["bs"],
                  "cpu_base_rmq_per_s": m / base_dt, "cpu_base_queries": m, "cpu_base_mismatches": base_bad,
                  "hrmq_rmq_per_s": Q1.shape[0] / h1_dt, "gpu_vs_hrmq": cmp}
-        log(f"cpu_baseline: config 1 GPU {config1['rmq_per_s']:.3e} RMQ/s, CPU BASE {m / base_dt:.3e} RMQ/s "
+        log(f"cpu_baseline: plain n=2^20 q=2^20: GPU {plain_small['rmq_per_s']:.3e} RMQ/s, CPU BASE {m / base_dt:.3e} RMQ/s "
             f"({m} queries, {base_bad} mismatches), HRMQ {Q1.shape[0] / h1_dt:.3e} RMQ/s, GPU vs HRMQ {cmp}")
         if base_bad or cmp["value_mismatches"] or cmp["index_mismatches"]:
-            raise SystemExit("cpu_baseline: config 1 answers differ")
+            raise SystemExit("cpu_baseline: plain n=2^20 q=2^20 answers differ")
     return {
         "value": total_q / total_t,
         "unit": "RMQ/s",
@@ -618,7 +618,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
         "port_construction_s": port_build_s,
         "port_bytes": port_size,
         "per_dist": per,
-        "config1": small,
+        "plain_small": small,
     }
 
 
@@ -1088,7 +1088,7 @@ def run_stress(R, torch, dev, args) -> dict:
     return {"n": n, "q": q, "tune_sample": sample, "steps": steps, "workloads": out}
 
 
-def run_config1(R, torch, dev, args) -> dict:
+def run_plain_small(R, torch, dev, args) -> dict:
     """The small configuration: n=2^20 U[0,1), 2^20 uniform-length queries,
     the literal bs=2^15, values and indices, timed on the GPU; its arrays
     and answers go to the host checks of the CPU baseline."""
@@ -1102,7 +1102,7 @@ def run_config1(R, torch, dev, args) -> dict:
         torch.cuda.synchronize()
         out = {"A": A1.cpu().numpy(), "Q": Q1, "val": v.cpu().numpy(), "idx": i.cpu().numpy(),
                "rmq_per_s": q1 / (float(np.mean(ms)) / 1e3), "bs": bs1}
-    log(f"config 1: n={n1} q={q1} bs={bs1}: {out['rmq_per_s']:.3e} RMQ/s")
+    log(f"plain small: n={n1} q={q1} bs={bs1}: {out['rmq_per_s']:.3e} RMQ/s")
     return out
 
 
@@ -1272,8 +1272,8 @@ def main() -> int:
     cpu = None
     del levels
     if not args.no_cpu_baseline:
-        config1 = run_config1(R, torch, dev, args)
-        cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds, gpu_answers, config1)
+        plain_small = run_plain_small(R, torch, dev, args)
+        cpu = cpu_baseline(A.cpu().numpy(), Q_host, args.cpu_seconds, gpu_answers, plain_small)
 
     attach_measured(tuned, workload, q)
     attach_measured(fixed, fixed_workload, q)

@@ -2,7 +2,7 @@
 
 Bar: bit-exact.  Values are compared as int32 bit patterns (so -0.0 vs +0.0
 counts), indices exactly, leftmost tie rule (CPU BASE, src/cpu_methods.h:7-53).
-Small cases go against brute force and the golden fixtures; the BASELINE
+Small cases go against brute force and the golden fixtures; the headline
 sizes (n=1e8, q=2^26; n=2^27 stress arrays) go against the O(1) fast oracle
 plus size-independent properties (l <= idx <= r, A[idx] == value).
 """
@@ -235,7 +235,7 @@ def test_autotuner_on_arrays_smaller_than_every_block_size(n, gpu):
     assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, f"n={n}")
 
 
-def test_config1_plain_small(gpu):
+def test_plain_small_n2e20(gpu):
     """n=2^20 U[0,1), q=2^20 uniform lengths, bs=2^15: engine vs fast oracle
     (all queries), vs CPU BASE brute force and GPU BASE (subsample), vs the
     reference HRMQ values when it is built."""
@@ -248,12 +248,12 @@ def test_config1_plain_small(gpu):
     f = O.FastOracle(A, nt=0)
     fv, fi = f.query(Q)
     f.close()
-    assert_exact(v, i, fv, fi, "config1 vs fast oracle")
+    assert_exact(v, i, fv, fi, "n=2^20 vs fast oracle")
     sub = np.random.default_rng(0).choice(q, 3000, replace=False)
     bv, bi = O.rmq_brute(A, Q[sub])
-    assert_exact(v[sub], i[sub], bv, bi, "config1 vs CPU BASE")
+    assert_exact(v[sub], i[sub], bv, bi, "n=2^20 vs CPU BASE")
     gv, gi = R.gpu_base(A_dev, _dev(Q[sub], gpu))
-    assert_exact(v[sub], i[sub], gv.cpu().numpy(), gi.cpu().numpy(), "config1 vs GPU BASE")
+    assert_exact(v[sub], i[sub], gv.cpu().numpy(), gi.cpu().numpy(), "n=2^20 vs GPU BASE")
     if O.HRMQ.available():
         h = O.HRMQ(A)
         hv, _ = h.query(Q)
