@@ -107,15 +107,36 @@ def _close_all() -> None:
                     pass
 
 
-def _finalizing() -> bool:
-    return sys.is_finalizing()
+class _Native:
+    """Closing shared by the native-object wrappers: close() releases the
+    object, a `with` block closes it at its end, and the finalizer closes it
+    unless the interpreter is finalizing.  The finalizer holds its own
+    reference to sys.is_finalizing: module globals may already be None when
+    an object is collected during teardown."""
+
+    def close(self):  # pragma: no cover -- every subclass defines it
+        raise NotImplementedError
+
+    def __del__(self, _is_finalizing=sys.is_finalizing):
+        if _is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def query_opts(short_max: int = 0, group_lanes: int = 0, long_wgs: int = 0, long_lanes: int = 0) -> N.QueryOpts:
     return N.QueryOpts(short_max, group_lanes, long_wgs, long_lanes)
 
 
-class RMQIndex:
+class RMQIndex(_Native):
     """A built block index over a device array (rmq_build / rmq_query / rmq_free).
 
     The array tensor is kept referenced: the edge-block scans read it.
@@ -195,22 +216,8 @@ class RMQIndex:
             N.lib().rmq_free(self._h)
             self._h = ctypes.c_void_p()
 
-    def __del__(self):
-        if _finalizing():
-            return
-        try:
-            self.close()
-        except Exception:  # noqa: BLE001
-            pass
 
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
-
-
-class HostHRMQ:
+class HostHRMQ(_Native):
     """The host HRMQ of algs 1/101 (rmq_hrmq_*): rightmost minimum index and
     its value bits, over values compared as int32 (a float array is read as
     its bit patterns, src/main.cu:117-125).  Runs on the CPU only."""
@@ -244,20 +251,6 @@ class HostHRMQ:
         if getattr(self, "_h", None) is not None and self._h.value:
             N.lib().rmq_hrmq_free(self._h)
             self._h = ctypes.c_void_p()
-
-    def __del__(self):
-        if _finalizing():
-            return
-        try:
-            self.close()
-        except Exception:  # noqa: BLE001
-            pass
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
 
 
 def handle_bytes(n: int, bs: int, flags: int = 0) -> int:
@@ -371,7 +364,7 @@ def gen_queries_device(q: int, n: int, lr: int, seed: int, device="cuda", stream
     return Q
 
 
-class Plan:
+class Plan(_Native):
     """Many arrays (or requests) answered by one persistent launch (rmq_plan_*).
 
     indexes[i] answers Qs[i] into out_idx[i] and/or out_val[i].  The tensors
@@ -425,22 +418,8 @@ class Plan:
             N.lib().rmq_plan_free(self._h)
             self._h = ctypes.c_void_p()
 
-    def __del__(self):
-        if _finalizing():
-            return
-        try:
-            self.close()
-        except Exception:  # noqa: BLE001
-            pass
 
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
-
-
-class Server:
+class Server(_Native):
     """Continuous-batching request loop over resident arrays (rmq_server_*).
 
     add_array keeps the array tensor referenced; submit keeps the request's
@@ -518,20 +497,6 @@ class Server:
             N.lib().rmq_server_destroy(self._h)
             self._h = ctypes.c_void_p()
             self._live.clear()
-
-    def __del__(self):
-        if _finalizing():
-            return
-        try:
-            self.close()
-        except Exception:  # noqa: BLE001
-            pass
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
 
 
 def gen_array_host(n: int, seed: int, kind: str = "uniform") -> np.ndarray:

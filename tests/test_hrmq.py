@@ -113,3 +113,49 @@ def test_same_answers_as_the_reference_library(kind, lr):
     ref.close()
     np.testing.assert_array_equal(i, ri)
     np.testing.assert_array_equal(v.view(np.int32), rv.view(np.int32))
+
+
+def test_objects_alive_at_interpreter_exit(tmp_path):
+    """Objects still referenced when the interpreter exits (one closed by its
+    `with` block, one never closed, both in reference cycles so that only the
+    last collection frees them) leave no finalizer error behind: the exit hook
+    closes the open one, and finalizers that run after module globals are
+    cleared do nothing."""
+    import subprocess
+    import sys
+
+    from conftest import ROOT
+    script = tmp_path / "exit_alive.py"
+    script.write_text(
+        "import sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "import numpy as np\n"
+        "import rtxrmq_amd as R\n"
+        "A = np.arange(100, dtype=np.float32)\n"
+        "with R.HostHRMQ(A) as closed:\n"
+        "    closed.query(np.array([[0, 99]], np.int32))\n"
+        "closed.cycle = closed\n"          # collected only by the last collection, after modules are cleared
+        "still_open = R.HostHRMQ(A)\n"
+        "still_open.cycle = still_open\n"
+        "print('done', flush=True)\n")
+    p = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.strip() == "done"
+    assert "Exception ignored" not in p.stderr and "Traceback" not in p.stderr, p.stderr
+
+
+def test_finalizer_after_module_globals_cleared():
+    """Interpreter teardown sets a module's globals to None before the last
+    objects are collected; a finalizer that runs then must not raise (the
+    failure was "'NoneType' object is not callable" from RMQIndex.__del__)."""
+    from rtxrmq_amd import engine as E
+    h = R.HostHRMQ(np.arange(10, dtype=np.float32))
+    h.close()
+    saved = {k: v for k, v in vars(E).items() if not k.startswith("__")}
+    try:
+        for k in saved:
+            setattr(E, k, None)
+        h.__del__()
+    finally:
+        for k, v in saved.items():
+            setattr(E, k, v)
