@@ -1145,6 +1145,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--no-gather-probe", action="store_true", help="skip the random-gather ceiling probe")
+    ap.add_argument("--gather-sweep", default=None, metavar="MIB,MIB,...",
+                    help="only time rmq_probe_gather over two tables of each size (MiB, powers of two) and exit")
     ap.add_argument("--reduce-pmc", default=None, metavar="DIR",
                     help="CPU only: turn rocprofv3 FETCH_SIZE/WRITE_SIZE runs under DIR into profiles/traffic.json")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30, help="size the --copy-probe run used")
@@ -1162,6 +1164,16 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copy probe: 3 copies of {nel * 4} bytes")
+        return 0
+
+    if args.gather_sweep:
+        # random 8-byte gathers against the footprint they are spread over:
+        # the request rate of lines from the last-level cache and from DRAM
+        import rtxrmq_amd as R
+        for mib in (int(x) for x in args.gather_sweep.split(",")):
+            r = R.probe_gather(table_bytes=mib << 20, count=1 << 26, reps=2, device="cuda")
+            print(json.dumps({"tables_mib": [mib, mib], "footprint_bytes": 2 * (mib << 20),
+                              "ms_per_launch": r["ms_per_launch"], "gathers_per_s": r["gathers_per_s"]}), flush=True)
         return 0
 
     if args.gpus != 1:
