@@ -278,7 +278,9 @@ int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t n, int32
 int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, int64_t q, int32_t *h_idx,
                       float *h_val, uint64_t *ticket);
 /* Blocks until the request is answered; returns its status (RMQ_ERR_OOM if it
- * can never fit the budget) and forgets the ticket. */
+ * can never fit the budget).  Several threads may wait on one ticket: each
+ * gets its status, and the ticket is forgotten when the last of them returns
+ * (a later wait on it returns RMQ_ERR_INVALID_ARG). */
 int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_ms);
 /* Latency percentiles and throughput since the last reset (reset != 0 clears). */
 int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset);

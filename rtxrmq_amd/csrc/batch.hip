@@ -443,6 +443,7 @@ struct Request {
     double t_submit = 0, t_done = 0;
     int status = RMQ_OK;
     bool done = false;
+    int waiters = 0;  // threads in rmq_server_wait for this ticket
 };
 
 struct Slot {
@@ -906,10 +907,14 @@ RMQ_API int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_m
     std::unique_lock<std::mutex> lk(sv->mu);
     auto it = sv->reqs.find(ticket);
     if (it == sv->reqs.end()) return RMQ_ERR_INVALID_ARG;
-    sv->cv_done.wait(lk, [&] { return it->second.done; });
-    const int status = it->second.status;
-    if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
-    sv->reqs.erase(it);
+    // a reference, not the iterator: submits during the wait may rehash the
+    // map, which moves no element but invalidates iterators
+    Request &r = it->second;
+    ++r.waiters;
+    sv->cv_done.wait(lk, [&] { return r.done; });
+    const int status = r.status;
+    if (latency_ms) *latency_ms = r.t_done - r.t_submit;
+    if (--r.waiters == 0) sv->reqs.erase(ticket);  // the last waiter forgets the ticket
     return status;
 }
 

@@ -196,3 +196,36 @@ def test_arrays_added_while_requests_run(arrays, gpu):
         f = O.FastOracle(A)
         np.testing.assert_array_equal(o, f.query(Q)[1], err_msg=f"added array {k}")
         f.close()
+
+
+def test_several_waiters_on_one_ticket_while_submits_rehash(arrays, gpu):
+    """Three threads wait on one ticket while the main thread submits
+    thousands of small requests (the ticket map rehashes many times under
+    the waiters): each waiter gets the status, a later wait on the ticket is
+    refused, and every answer equals the oracle's."""
+    import threading
+    A, Ad = arrays[2]
+    with R.Server(slots=2) as sv:
+        aid = sv.add_array(Ad, 256, R.BUILD_EDGES)
+        Qb = O.gen_queries(1 << 18, A.size, -1, 77)
+        big_i = np.full(Qb.shape[0], -5, np.int32)
+        tk = sv.submit(aid, Qb, out_idx=big_i)
+        got = []
+        waiters = [threading.Thread(target=lambda: got.append(sv.wait(tk)[0])) for _ in range(3)]
+        for w in waiters:
+            w.start()
+        small = []
+        for k in range(3000):
+            Q = O.gen_queries(16, A.size, -3, 9000 + k)
+            oi = np.full(16, -5, np.int32)
+            small.append((Q, oi, sv.submit(aid, Q, out_idx=oi)))
+        for w in waiters:
+            w.join(timeout=60)
+        assert not any(w.is_alive() for w in waiters)
+        assert got == [R.RMQ_OK] * 3
+        assert sv.wait(tk)[0] == R.RMQ_ERR_INVALID_ARG  # forgotten once its last waiter returned
+        for Q, oi, t in small:
+            assert sv.wait(t)[0] == R.RMQ_OK
+    np.testing.assert_array_equal(big_i, O.rmq_brute(A, Qb)[1])
+    for Q, oi, _ in small:
+        np.testing.assert_array_equal(oi, O.rmq_brute(A, Q)[1])
