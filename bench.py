@@ -645,6 +645,17 @@ def timed_run(R, torch, index_for, opts_for, Q_dev, out_v, out_i, steps: int, wa
     return steps_out, time.perf_counter() - t_start
 
 
+def answers_digest(torch, index_for, opts_for, Q_dev, out_v, out_i) -> dict:
+    """Per batch, (sum of the indexes, sum of the value bits as int32) of one
+    more pass, computed on the device: equal digests of two configurations
+    over the same queries say their answers agree."""
+    out = {}
+    for lr in DISTS:
+        index_for[lr].query(Q_dev[lr], out_val=out_v, out_idx=out_i, opts=opts_for[lr])
+        out[lr] = (int(out_i.to(torch.int64).sum()), int(out_v.view(torch.int32).to(torch.int64).sum()))
+    return out
+
+
 def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_for=None, kernels_for=None) -> dict:
     """RMQ/s, per-distribution figures and the roofline of the dominant kernel.
     levels: BlockLevel per block size (and 32, with edge tables); kernels_for:
@@ -1145,6 +1156,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     ap.add_argument("--no-gather-probe", action="store_true", help="skip the random-gather ceiling probe")
+    ap.add_argument("--no-fixed-tuned", action="store_true",
+                    help="skip the run at --bs with edge tables and thresholds from the tuner restricted to it")
     ap.add_argument("--gather-sweep", default=None, metavar="MIB,MIB,...",
                     help="only time rmq_probe_gather over two tables of each size (MiB, powers of two) and exit")
     ap.add_argument("--reduce-pmc", default=None, metavar="DIR",
@@ -1212,6 +1225,8 @@ def main() -> int:
     levels = {args.bs: BlockLevel(A, args.bs)}
     fixed = summarise(A, Q_dev, q, fixed_steps, fixed_elapsed, levels, {lr: args.bs for lr in DISTS},
                       {lr: fixed_opts for lr in DISTS})
+    fixed_digest = answers_digest(torch, {lr: fixed_index for lr in DISTS}, {lr: fixed_opts for lr in DISTS}, Q_dev,
+                                  out_v, out_i)
     fixed_index.close()
 
     # --- autotuned configuration per distribution ----------------------------------
@@ -1251,6 +1266,46 @@ def main() -> int:
     kernels_for = {lr: index_for[lr].kernels(opts_for[lr]) for lr in DISTS}
     tuned = summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for,
                       {lr: picks[lr]["flags"] for lr in DISTS}, kernels_for)
+    tuned_digest = answers_digest(torch, index_for, opts_for, Q_dev, out_v, out_i)
+
+    # --- the literal block size with the tuner's structure --------------------------
+    # rmq_tune restricted to bs = args.bs picks edge tables and thresholds per
+    # distribution (its picks are cached under keys of their own)
+    fixed_tuned = None
+    if not args.no_fixed_tuned:
+        fpicks = {}
+        for lr in DISTS:
+            sample = R.gen_queries_device(min(q, args.tune_sample), n, lr, args.seed + 7919 + 17 * (-lr), device=dev)
+            fpicks[lr] = R.tune(A, sample, bs_list=[args.bs], cache_path=args.tune_cache or None, refresh=args.retune)
+            del sample
+        f_indexes = {}
+        for lr in DISTS:
+            if fpicks[lr]["flags"] not in f_indexes:
+                f_indexes[fpicks[lr]["flags"]] = R.RMQIndex(A, args.bs, flags=fpicks[lr]["flags"])
+        f_index_for = {lr: f_indexes[fpicks[lr]["flags"]] for lr in DISTS}
+        f_opts = {lr: fpicks[lr]["opts"] for lr in DISTS}
+        f_steps, f_elapsed = timed_run(R, torch, f_index_for, f_opts, Q_dev, out_v, out_i, args.fixed_steps, 1,
+                                       f"bs={args.bs} tuned")
+        if any(p["flags"] for p in fpicks.values()) and 32 not in levels:
+            levels[32] = BlockLevel(A, 32)
+        fixed_tuned = summarise(A, Q_dev, q, f_steps, f_elapsed, levels, {lr: args.bs for lr in DISTS}, f_opts,
+                                {lr: fpicks[lr]["flags"] for lr in DISTS},
+                                {lr: f_index_for[lr].kernels(f_opts[lr]) for lr in DISTS})
+        f_digest = answers_digest(torch, f_index_for, f_opts, Q_dev, out_v, out_i)
+        fixed_tuned["picks"] = {DIST_NAMES[lr]: {"edge_tables": bool(fpicks[lr]["flags"]),
+                                                 "short_max": f_opts[lr].short_max,
+                                                 "group_lanes": f_opts[lr].group_lanes,
+                                                 "long_lanes": f_opts[lr].long_lanes,
+                                                 "source": "cache" if fpicks[lr]["from_cache"] else "timed"}
+                                for lr in DISTS}
+        fixed_tuned["identical_to_tuned"] = f_digest == tuned_digest
+        fixed_tuned["workload"] = workload_key(n, q, {lr: args.bs for lr in DISTS}, f_opts,
+                                               {lr: fpicks[lr]["flags"] for lr in DISTS})
+        fixed_tuned["steps"] = args.fixed_steps
+        for ix in f_indexes.values():
+            ix.close()
+        log(f"bench: bs={args.bs} with the tuner's structure {fixed_tuned['value']:.4e} RMQ/s "
+            f"(literal {fixed['value']:.4e}); answers identical to the tuned picks: {fixed_tuned['identical_to_tuned']}")
     log(f"bench: tuned {tuned['value']:.4e} RMQ/s; {tuned['dominant']} {tuned['achieved']:.1f} GB/s algorithmic "
         f"({tuned['achieved'] / HBM_PEAK_GBS:.3f} of peak); fixed bs={args.bs}: {fixed['value']:.4e} RMQ/s")
     for ix in indexes.values():
@@ -1345,8 +1400,25 @@ def main() -> int:
                          "traffic": fixed_traffic,
                          "traffic_GBps": fixed_traffic / (fixed_ms / 1e3) / 1e9 if fixed_traffic else None,
                          "avg_launch_ms": fixed_ms,
-                         "workload": fixed_workload}},
+                         "workload": fixed_workload},
+            "identical_to_tuned": fixed_digest == tuned_digest},
     }
+    if fixed_tuned is not None:
+        ft_workload = fixed_tuned["workload"]
+        attach_measured(fixed_tuned, ft_workload, q)
+        ft_ms = fixed_tuned["per_kernel"][fixed_tuned["dominant"]]["avg_launch_ms"]
+        ft_traffic = load_traffic(fixed_tuned["dominant"], ft_workload)
+        line[f"fixed_bs_{args.bs}_tuned"] = {
+            "value": fixed_tuned["value"], "ms_per_step": fixed_tuned["ms_per_step"], "steps": fixed_tuned["steps"],
+            "picks": fixed_tuned["picks"], "identical_to_tuned": fixed_tuned["identical_to_tuned"],
+            "per_dist": fixed_tuned["per_dist"], "per_kernel": fixed_tuned["per_kernel"],
+            "roofline": {"kernel": fixed_tuned["dominant"], "achieved": fixed_tuned["achieved"],
+                         "frac": fixed_tuned["achieved"] / HBM_PEAK_GBS,
+                         "achieved_model": fixed_tuned["achieved_model"],
+                         "frac_model": fixed_tuned["achieved_model"] / HBM_PEAK_GBS,
+                         "traffic": ft_traffic,
+                         "traffic_GBps": ft_traffic / (ft_ms / 1e3) / 1e9 if ft_traffic else None,
+                         "avg_launch_ms": ft_ms, "workload": ft_workload}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -164,8 +164,8 @@ typedef struct rmq_tune_result {
  *      (NULL: no cache) under a key of: the tuner's version, log2 n and the
  *      10/50/90th percentiles of log2(range length) of the sample, the
  *      array's value signature (mean position of the leftmost minimum in
- *      sampled 64-element windows, and a tie-density class) and the outputs
- *      timed ("vi", "v" or "i").  d_val / d_idx are scratch outputs of q
+ *      sampled 64-element windows, and a tie-density class), the outputs
+ *      timed ("vi", "v" or "i") and, when bs_list is given, that list.  d_val / d_idx are scratch outputs of q
  *      entries (one may be NULL: pass the outputs the real queries will
  *      ask for).  Synchronising; builds and frees its own handles. ---- */
 int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,

@@ -20,6 +20,7 @@
 #include <atomic>
 #include <climits>
 #include <cstdio>
+#include <string>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1172,7 +1173,22 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     int rc = tune_signature(d_Q, q, n, s, lens, sizeof(lens));
     if (rc == RMQ_OK) rc = value_signature(d_A, n, s, vals, sizeof(vals));
     if (rc != RMQ_OK) return rc;
-    snprintf(key, sizeof(key), "%s/%s/%s/out=%s", kTuneVersion, lens, vals, d_val && d_idx ? "vi" : (d_val ? "v" : "i"));
+    // a caller's candidate list is part of the key (its pick answers a
+    // different question): the list itself when short, else its FNV-1a hash
+    char cands_tag[40] = "";
+    if (n_bs > 0) {
+        std::string list;
+        for (int32_t i = 0; i < n_bs; ++i) list += (i ? "," : "") + std::to_string(bs_list[i]);
+        if (list.size() <= 24) {
+            snprintf(cands_tag, sizeof(cands_tag), "/bs=%s", list.c_str());
+        } else {
+            uint32_t hsh = 2166136261u;
+            for (char ch : list) hsh = (hsh ^ (uint8_t)ch) * 16777619u;
+            snprintf(cands_tag, sizeof(cands_tag), "/bs#%08x", hsh);
+        }
+    }
+    snprintf(key, sizeof(key), "%s/%s/%s/out=%s%s", kTuneVersion, lens, vals, d_val && d_idx ? "vi" : (d_val ? "v" : "i"),
+             cands_tag);
     if (strlen(key) >= sizeof(out->key)) return RMQ_ERR_INVALID_ARG;
     if (cache_lookup(cache_path, key, out)) {
         out->from_cache = 1;

@@ -181,7 +181,19 @@ def test_autotuner_pick_and_cache(tmp_path, gpu):
     assert r1["opts"].group_lanes in (4, 8, 16, 64) and r1["opts"].long_lanes in (4, 8, 16, 64)
     r2 = R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)
     assert r2["from_cache"] and r2["bs"] == r1["bs"] and r2["key"] == r1["key"]
-    assert r1["key"].startswith("t2/n2^22.0/") and r1["key"].endswith("/pos0.50/tie0/out=vi"), r1["key"]
+    assert r1["key"].startswith("t2/n2^22.0/") and r1["key"].endswith("/pos0.50/tie0/out=vi/bs=256,1024,4096"), \
+        r1["key"]
+    # another candidate list asks another question: a key of its own, timed afresh
+    r3 = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache)
+    assert not r3["from_cache"] and r3["bs"] == 1024 and r3["key"].endswith("/out=vi/bs=1024"), r3["key"]
+    # a list longer than 24 characters is keyed by the FNV-1a hash of its text
+    long_list = [1024, 2048, 4096, 8192, 16384]
+    h = 2166136261
+    for ch in ",".join(map(str, long_list)).encode():
+        h = ((h ^ ch) * 16777619) & 0xFFFFFFFF
+    r4 = R.tune(A_dev, Q_dev, bs_list=long_list, cache_path=cache)
+    assert not r4["from_cache"] and r4["key"].endswith(f"/out=vi/bs#{h:08x}"), r4["key"]
+    assert R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)["bs"] == r1["bs"]
     with R.RMQIndex(A_dev, r1["bs"]) as ix:
         v, i = ix.query(Q_dev, opts=r1["opts"])
         torch.cuda.synchronize()
@@ -200,13 +212,13 @@ def test_tuner_key_separates_value_kinds_and_outputs(tmp_path, gpu):
     for kind in ("uniform", "ties16", "desc"):
         A_dev = R.gen_array(n, 9, kind, device=gpu)
         keys[kind] = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache)["key"]
-    assert keys["uniform"].endswith("/pos0.50/tie0/out=vi"), keys
+    assert keys["uniform"].endswith("/pos0.50/tie0/out=vi/bs=1024"), keys
     assert "/tie2/" in keys["ties16"] and "/pos0.25/" in keys["ties16"], keys
-    assert keys["desc"].endswith("/pos1.00/tie0/out=vi"), keys
+    assert keys["desc"].endswith("/pos1.00/tie0/out=vi/bs=1024"), keys
     A_dev = R.gen_array(n, 9, "uniform", device=gpu)
     r_i = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache, want_value=False)
     r_v = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache, want_index=False)
-    assert r_i["key"].endswith("/out=i") and r_v["key"].endswith("/out=v")
+    assert r_i["key"].endswith("/out=i/bs=1024") and r_v["key"].endswith("/out=v/bs=1024")
     assert not r_i["from_cache"] and not r_v["from_cache"]
 
 
@@ -316,6 +328,39 @@ def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
     fv, fi = f.query(Q)
     f.close()
     assert_exact(v, i, fv, fi, f"lr={lr} pick={pick['key']} bs={pick['bs']} kernels={kern}")
+
+
+# The kernels the committed picks restricted to bs = 2^15 run (edge tables for all three).
+LITERAL_BS_KERNELS = {-1: ["k_query_edges<4, false>"], -2: ["k_query_edges<16, false>"],
+                      -3: ["k_query_edges<16, false>"]}
+
+
+@pytest.mark.parametrize("lr", [-1, -2, -3])
+def test_committed_picks_at_bs_2_15_full_size(lr, tmp_path, gpu):
+    """The headline batches at the stated block size 2^15 with the structure
+    the tuner restricted to it picks (edge tables and thresholds, from the
+    committed cache), every answer checked against the O(1) oracle."""
+    import shutil
+    n, q = 100_000_000, 1 << 26
+    A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
+    Q = R.gen_queries(q, n, lr, 1321 + lr)
+    Q_dev = _dev(Q, gpu)
+    cache = tmp_path / "picks.txt"
+    shutil.copy(os.path.join(os.path.dirname(R.__file__), "tune_cache_mi355x.txt"), cache)
+    pick = R.tune(A_dev, Q_dev[: 1 << 22], bs_list=[1 << 15], cache_path=str(cache))
+    assert pick["from_cache"] and pick["bs"] == 1 << 15, f"lr={lr}: key {pick['key']} missed the committed cache"
+    with R.RMQIndex(A_dev, pick["bs"], flags=pick["flags"]) as ix:
+        kern = ix.kernels(pick["opts"])
+        assert kern == LITERAL_BS_KERNELS[lr], f"lr={lr}: {kern}"
+        v, i = ix.query(Q_dev, opts=pick["opts"])
+        torch.cuda.synchronize()
+        v, i = v.cpu().numpy(), i.cpu().numpy()
+    del Q_dev
+    A = A_dev.cpu().numpy()
+    f = O.FastOracle(A)
+    fv, fi = f.query(Q)
+    f.close()
+    assert_exact(v, i, fv, fi, f"lr={lr} bs=2^15 pick={pick['key']} kernels={kern}")
 
 
 @pytest.mark.parametrize("flags", FLAGS)
