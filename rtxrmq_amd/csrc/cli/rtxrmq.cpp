@@ -59,6 +59,7 @@ struct Args {
     bool qgen_ref = false;  // the reference's own query stream (src/rand.cuh:116-201)
     bool edges = false;     // build edge tables (RMQ_BUILD_EDGES)
     bool tune = false;      // pick bs, edge tables and thresholds with rmq_tune
+    bool bs_given = false;  // --bs or --nb on the command line: --tune keeps that block size
     std::string tune_cache;
 };
 
@@ -89,7 +90,8 @@ void usage() {
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
             "  --edges               build edge tables: O(1) answers for ranges not inside a 32-element sub-block\n"
             "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^24 queries\n"
-            "                        (overrides --bs/--nb/--edges and the lane options; optional pick cache)\n"
+            "                        (overrides --edges and the lane options; with --bs/--nb the block size\n"
+            "                        stays and only edge tables and thresholds are tuned; optional pick cache)\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
             "  --qgen <gen>          ours (default: counter-based, thread-count independent) |\n"
             "                        ref (the reference's mt19937(seed*tid) stream; depends on --nt;\n"
@@ -134,8 +136,8 @@ bool parse(int argc, char **argv, Args &a) {
     optind = 5;
     while ((opt = getopt_long(argc, argv, "", opts, &idx)) != -1) {
         switch (opt) {
-        case O_BS: a.bs = std::max(1, std::min(a.n, atoi(optarg))); a.nb = a.n / a.bs; break;
-        case O_NB: a.nb = std::max(1, std::min(a.n, atoi(optarg))); a.bs = a.n / a.nb; break;
+        case O_BS: a.bs = std::max(1, std::min(a.n, atoi(optarg))); a.nb = a.n / a.bs; a.bs_given = true; break;
+        case O_NB: a.nb = std::max(1, std::min(a.n, atoi(optarg))); a.bs = a.n / a.nb; a.bs_given = true; break;
         case O_REPS: a.reps = std::max(1, atoi(optarg)); break;
         case O_DEV: a.dev = atoi(optarg); break;
         case O_NT: a.nt = std::max(1, atoi(optarg)); break;
@@ -335,7 +337,9 @@ int main(int argc, char **argv) {
         fflush(stdout);
         t0 = now_ms();
         rmq_tune_result_t pick{};
-        if (int rc = rmq_tune(dA, a.n, dQ, sample, nullptr, 0, idx_out ? nullptr : d_sv, idx_out ? d_si : nullptr,
+        const int32_t only_bs = a.bs;  // the block size given with --bs/--nb, the only candidate then
+        if (int rc = rmq_tune(dA, a.n, dQ, sample, a.bs_given ? &only_bs : nullptr, a.bs_given ? 1 : 0,
+                              idx_out ? nullptr : d_sv, idx_out ? d_si : nullptr,
                               a.tune_cache.empty() ? nullptr : a.tune_cache.c_str(), nullptr, &pick))
             die("rmq_tune", rc);
         (void)hipFree(d_sv);

@@ -146,6 +146,19 @@ def test_gpu_edges_and_autotune(alg, tmp_path, gpu):
 
 
 @pytest.mark.gpu
+def test_gpu_autotune_keeps_a_given_block_size(tmp_path, gpu):
+    """--tune with --nb (the reference's sweep parameter) tunes edge tables and
+    thresholds at that block size only; the row and the pick carry it."""
+    n, nb = 1 << 20, 64
+    csv = tmp_path / "t.csv"
+    r = run(n, 1 << 16, -3, 105, "--nb", nb, "--tune", "--reps", 2, "--seed", 6, "--check", f"--save-time={csv}")
+    assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
+    m = re.search(r"\[bs=(\d+) edges=(\d)", r.stdout)
+    assert m and int(m.group(1)) == n // nb, r.stdout[-1500:]
+    assert csv.read_text().strip().split(",")[4] == str(n // nb)
+
+
+@pytest.mark.gpu
 def test_gpu_save_power_writes_reference_columns(tmp_path, gpu):
     """--save-power samples the amdgpu hwmon sensor every 100 ms during the
     query loop into the reference's six columns (common/nvmlPower.cpp:37-84):
