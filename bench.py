@@ -1348,6 +1348,15 @@ def main() -> int:
     dom_ms = tuned["per_kernel"][tuned["dominant"]]["avg_launch_ms"]
     fixed_traffic = load_traffic(fixed["dominant"], fixed_workload)
     fixed_ms = fixed["per_kernel"][fixed["dominant"]]["avg_launch_ms"]
+    if gather_ceiling is not None:
+        # the dominant kernel's measured read requests past the L2 (calibrated
+        # counters of this workload) per second of its launches, against the
+        # probe's rate of random requests from 1 GiB tables
+        dom_dist = [d for d in tuned["per_dist"].values() if tuned["dominant"] in d["kernels"].values()]
+        if len(dom_dist) == 1 and "dram_read_lines_per_rmq" in dom_dist[0]:
+            req = dom_dist[0]["dram_read_lines_per_rmq"] * q / (dom_ms / 1e3)
+            gather_ceiling["kernel_read_requests_per_s"] = req
+            gather_ceiling["read_requests_over_probe"] = req / gather_ceiling["dram_gathers_per_s"]
     line = {
         "metric": "RMQ/s at n=1e8,q=2^26 per length dist, %HBM roofline; batched/served vs plain",
         "value": tuned["value"],
