@@ -192,9 +192,11 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     block's leftmost argmin lies inside it (then that entry is exact) or its
     block minimum cannot win (left: > the best so far; right: >= it); a needed
     part costs its scan's bytes (scan_bytes), or 8 B (one edge pair) with
-    edge tables.  two_stage (edge tables, large tables): a failed superset
-    test is first retried with the failing side cut off (16 B); when that
-    answers, one edge pair (8 B) completes it.  With edge tables the left pair, once read, counts in the best so
+    edge tables.  two_stage (edge tables, large tables; answer_two_stage): a
+    failed superset test is followed by the whole blocks of the side that did
+    not fail (16 B) and the failing side's edge pair (8 B); when that does not
+    answer, by the other edge pair (8 B) and the blocks between (16 B when
+    whole blocks lie between).  With edge tables the left pair, once read, counts in the best so
     far before the right part is decided; a scanned left part (spread over a
     lane group) does not.  T = 16 when whole blocks (or sub-blocks) lie
     between.  sub: the 32-element level and A the array, with edge tables.
@@ -240,24 +242,40 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         reads += int(sr.sum())
         x = ~fast
         l, r, lB, rB, sr, sup_i = l[x], r[x], lB[x], rB[x], sr[x], sup_i[x]
-        s2b = torch.zeros_like(l)  # bytes of a second superset test that did not answer
         if edges and two_stage:
-            # second test with the failed side cut off: argmin before l (blocks
-            # lB+1..rB, then the left edge pair) or in block rB after r (blocks
-            # lB..rB-1, then the right edge pair)
+            # answer_two_stage's phases 2 and 3 (phase 1 is the superset test
+            # above): the blocks of the side that did not fail, from the line
+            # just read when the two entries shared it, with the failing side's
+            # edge pair; failing again, both edge pairs and the blocks between
+            c = rB - lB + 1
+            lev = (torch.frexp(c.double())[1] - 1).long()
+            a2 = rB - torch.bitwise_left_shift(torch.ones_like(lev), lev) + 1
+            base = lev * lvl.nb
+            inl = (a2 == lB) | (((base + lB) * 8) // 128 == ((base + a2) * 8) // 128)
             left = sup_i < l
-            right = ~left & (sup_i < (rB + 1) * bs)
-            tried = left | right
-            _, s2i = lvl.range_min(torch.where(left, lB + 1, lB), rB - lB)
-            ok2 = (left & (s2i <= r)) | (right & (s2i >= l))
-            reads += 2 * int(tried.sum())
-            gathers += int(ok2.sum())
-            parts += 2 * int(ok2.sum())
-            needed += int(ok2.sum())
-            long_b += float((8 + 8 * sr[ok2] + 16 + 8 + out_bytes).sum())
-            x = ~ok2
-            l, r, lB, rB, sr = l[x], r[x], lB[x], rB[x], sr[x]
-            s2b = 16 * tried[x].long()
+            q1 = torch.where(left, lB + 1, lB)
+            q2 = torch.where(left, a2, a2 - 1)
+            adj = inl & (a2 == lB) & torch.where(left, rB + 1 < lvl.nb, lB > 0)
+            q1 = torch.where(adj, torch.where(left, lB + 1, lB - 1), q1)
+            q2 = torch.where(adj, q1, q2)
+            pair = inl & (a2 > lB)
+            gv, gi = lvl.range_min(q1, c - 1)  # the general window over c-1 blocks
+            # entries of the line (only lanes in pair | adj use them; the others' indexes are clamped)
+            q1c, q2c = q1.clamp(0, lvl.nb - 1), q2.clamp(0, lvl.nb - 1)
+            uv, ui = lvl.V[lev, q1c], lvl.I[lev, q1c]
+            vv, vi = lvl.V[lev, q2c], lvl.I[lev, q2c]
+            t = vv < uv
+            s2i = torch.where(pair | adj, torch.where(t, vi, ui), gi)
+            ok2 = torch.where(left, s2i <= r, s2i >= l)
+            n2, n3 = int(ok2.sum()), int((~ok2).sum())
+            mid = ~ok2 & (c > 2)
+            reads += 2 * (n2 + n3) + 2 * int(mid.sum())
+            gathers += n2 + 2 * n3
+            needed += n2 + 2 * n3
+            parts += 2 * (n2 + n3)
+            long_b += float((8 + 8 * sr + 16 + 8 + 8 * (~ok2).long() + 16 * mid.long() + out_bytes).sum())
+            continue
+        s2b = torch.zeros_like(l)
         mv = torch.full_like(l, 0, dtype=torch.float32).fill_(float("inf"))
         mi = torch.full_like(l, 2 ** 31 - 1)
         mid = rB - lB > 1

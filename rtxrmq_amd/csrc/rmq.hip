@@ -238,7 +238,8 @@ __global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pai
 // are not short (load_tile_edges) one lane per query by O(1) lookups
 // (answer_edges: no scan, no reduction), then the short ranges with G-lane
 // group scans (answer_class), and writes (NaN, -1) for invalid queries.
-// TWO: failed superset tests are retried (answer_edges), for large tables.
+// TWO: the phased superset test and failure path for tables beyond the
+// caches (answer_two_stage).
 template <int G, bool TWO>
 __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, EdgeTables e, const int2 *__restrict__ Q, int64_t q,
                                                      int short_max, float *__restrict__ out_v,
@@ -541,8 +542,8 @@ enum class LongPass { kTiles, kEdges, kEdgesLds };
 
 size_t lds_table_bytes(const rmq_handle *h) { return (size_t)h->levels * (size_t)h->nb * sizeof(Pair); }
 
-// Whether the global-table kernel retries failed superset tests: only for
-// tables far beyond the caches, where its saved DRAM lines pay.
+// Whether the global-table kernel takes the phased path of answer_two_stage:
+// only for tables far beyond the caches, where its saved DRAM lines pay.
 bool two_stage(const rmq_handle *h) { return h->st_bytes > kTwoStageTableBytes; }
 
 LongPass long_pass(const rmq_handle *h) {

@@ -400,14 +400,8 @@ struct EdgeTables {
 // bl..br from two level-lev entries, or, with LINES, when those two entries
 // lie in different 128-byte lines, the one level-(lev+1) entry at bl (blocks
 // bl..bl+2^(lev+1)-1, up to twice as many): one line instead of two.
-// RETRY: when that wider window's argmin lies past block br, it says nothing
-// about bl..br, and the two level-lev entries are read after all (a CPU
-// replay of the lognormal 2^6 batch at bs=8: 5% of ranges, which the
-// failure path would otherwise take; 1.40 -> 1.32 DRAM lines per range).  On
-// a table in L2 the extra dependent read costs more than it saves
-// (lognormal 2^14 at bs=1024: 0.93 -> 0.98 ms), so it goes with TWO_STAGE.
-template <bool LINES, bool RETRY>
-__device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb, int bs, int bl, int br) {
+template <bool LINES>
+__device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb, int bl, int br) {
     const int c = br - bl + 1;
     const int lev = 31 - __clz(c);
     const Pair *row = st + (int64_t)lev * nb;
@@ -415,46 +409,22 @@ __device__ __forceinline__ Pair superset_min(const Pair *__restrict__ st, int nb
     if (LINES && ((reinterpret_cast<uintptr_t>(row + bl) ^ reinterpret_cast<uintptr_t>(row + a2)) >> 7) != 0 &&
         (int64_t)bl + ((int64_t)2 << lev) <= nb) {
         const Pair w = row[nb + bl];  // level lev + 1 (it exists: 2^(lev+1) <= nb)
-        if (!RETRY || (int64_t)w.i < (int64_t)(br + 1) * bs) return w;
+        return w;
     }
     const Pair x = row[bl];
     const Pair y = row[a2];
     return (y.v < x.v) ? y : x;
 }
 
-// A range across blocks bl < br whose superset window minimum sup has its
-// argmin outside [l, r].
-template <bool NT_PAIRS, bool TWO_STAGE>
-__device__ __forceinline__ Pair answer_failed(const EdgeTables &e, const Blocks &k, int l, int r, int bl, int br,
-                                              Pair sup) {
+// A range across blocks bl < br whose superset window minimum has its argmin
+// outside [l, r]: whole blocks between, then each edge from its block's
+// (min, leftmost argmin) in table row 0: exact when the argmin lies in the
+// edge part, skipped when the block minimum cannot win, and only otherwise
+// read from the edge table.
+template <bool NT_PAIRS>
+__device__ __forceinline__ Pair answer_failed(const EdgeTables &e, const Blocks &k, int l, int r, int bl, int br) {
     const Pair *__restrict__ st = k.st;
-    const int nb = k.nb, bs = k.bs;
-    // Second test, with the side that failed cut off: the argmin lies in
-    // block bl before l (blocks bl+1..br are tried, plus the left edge
-    // pair) or in block br after r (blocks bl..br-1, plus the right edge
-    // pair).  The shrunk window's argmin inside [l, r] makes it the
-    // answer for the whole blocks, and the one edge pair completes it.
-    if constexpr (TWO_STAGE) {
-        if (sup.i < l) {
-            const Pair s2 = table_min(st, nb, bl + 1, br - bl);
-            if (s2.i <= r) {
-                Pair x;
-                if constexpr (NT_PAIRS) x = nt_load_pair(e.suf_b + l); else x = e.suf_b[l];
-                return (x.v <= s2.v) ? x : s2;  // the left part's indices are smaller: it wins ties
-            }
-        } else if ((int64_t)sup.i < (int64_t)(br + 1) * bs) {
-            const Pair s2 = table_min(st, nb, bl, br - bl);
-            if (s2.i >= l) {
-                Pair x;
-                if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
-                return (x.v < s2.v) ? x : s2;  // the right part's indices are larger: strictly smaller wins
-            }
-        }
-    }
-    // Whole blocks between, then each edge from its block's (min, leftmost
-    // argmin) in table row 0: exact when the argmin lies in the edge part,
-    // skipped when the block minimum cannot win, and only otherwise read
-    // from the edge table.
+    const int nb = k.nb;
     Pair best = br - bl > 1 ? table_min(st, nb, bl + 1, br - bl - 1) : Pair{INFINITY, INT_MAX};
     const Pair el = st[bl];
     if (el.i >= l) {
@@ -472,6 +442,110 @@ __device__ __forceinline__ Pair answer_failed(const EdgeTables &e, const Blocks 
         if constexpr (NT_PAIRS) x = nt_load_pair(e.pre_b + r); else x = e.pre_b[r];
         if (x.v < best.v) best = x;
     }
+    return best;
+}
+
+template <bool NT>
+__device__ __forceinline__ Pair load_pair(const Pair *p) {
+    if constexpr (NT) return nt_load_pair(p); else return *p;
+}
+
+// Whether two table entries lie in one 128-byte line.
+__device__ __forceinline__ bool one_line(const Pair *a, const Pair *b) {
+    return ((reinterpret_cast<uintptr_t>(a) ^ reinterpret_cast<uintptr_t>(b)) >> 7) == 0;
+}
+
+// A range across blocks bl < br of a table far beyond the caches (the
+// lognormal 2^6 batch's bs=8 table is 2.4 GB), where every line read is a
+// DRAM request, in three phases; in each every lane picks its addresses
+// first and the phase's loads go out together, so lanes on different paths
+// do not wait for each other's loads in turn.
+// 1. Superset test: the two level-lev entries (lev = floor(log2(br-bl+1)),
+//    at bl and a2 = br - 2^lev + 1) when they share a 128-byte line ("in
+//    line"; always for a single window, a2 == bl); else the one
+//    level-(lev+1) entry at bl when it fits ("wide"), read again as the two
+//    level-lev entries when its argmin lies past block br, where it says
+//    nothing about bl..br; else the two entries.
+// 2. The test failed on one side, so the other side's whole blocks and the
+//    failing side's edge pair may answer it: argmin before l -> blocks
+//    bl+1..br (in line: the entries at bl+1 and a2 of the line just read,
+//    or, when a2 == bl, the one window at bl+1, blocks bl+1..br+1), whose
+//    argmin <= r makes it the answer for the whole blocks, with the suffix
+//    pair of l loaded beside it; argmin in block br after r -> blocks
+//    bl..br-1 (bl-1..br-1) and the prefix pair of r.
+// 3. Otherwise the suffix pair of l, the blocks between (from the line when
+//    a2 >= bl + 2) and the prefix pair of r.
+// A CPU replay of the lognormal 2^6 batch counts 1.31 -> 1.28 DRAM lines per
+// range against a superset test followed by the general edge path; the same
+// GPU box ran the batch in 1.899 -> 1.858 ms, answers identical.  Earlier
+// steps: retrying a failed test with the failing side cut off took it from
+// 2.13 to 2.01 ms, the exact re-read of a wide window from 1.927 to 1.870.
+// A second copy of rows 1..4, 64 bytes out of line phase, so that the two
+// entries always share a line, cut DRAM requests by another 5% but pushed
+// the randomly read footprint past the translation caches' reach, and was
+// no faster.  On tables that stay in the caches the extra dependent reads
+// cost more than they save (lognormal 2^14 at bs=1024), so the host uses
+// this path only for tables over kTwoStageTableBytes.
+template <bool NT_PAIRS>
+__device__ __forceinline__ Pair answer_two_stage(const EdgeTables &e, const Blocks &k, int l, int r, int bl, int br) {
+    const Pair *__restrict__ st = k.st;
+    const int nb = k.nb, bs = k.bs;
+    const int c = br - bl + 1;
+    const int lev = 31 - __clz(c);
+    const int a2 = br - (1 << lev) + 1;
+    const Pair *row = st + (int64_t)lev * nb;
+    const bool inl = a2 == bl || one_line(row + bl, row + a2);
+    const bool wide = !inl && (int64_t)bl + ((int64_t)2 << lev) <= nb;
+    // phase 1
+    const Pair *p1 = wide ? row + nb + bl : row + bl;
+    const Pair *p2 = wide ? p1 : row + a2;
+    Pair x = *p1, y = *p2;
+    Pair sup = (y.v < x.v) ? y : x;
+    if (wide && (int64_t)sup.i >= (int64_t)(br + 1) * bs) {
+        x = row[bl];
+        y = row[a2];
+        sup = (y.v < x.v) ? y : x;
+    }
+    if (sup.i >= l && sup.i <= r) return sup;
+    // phase 2
+    const bool left = sup.i < l;
+    const Pair *q1, *q2;
+    if (inl && a2 > bl) {
+        q1 = left ? row + bl + 1 : row + bl;
+        q2 = left ? row + a2 : row + a2 - 1;
+    } else if (inl && left && br + 1 < nb) {
+        q1 = q2 = row + bl + 1;
+    } else if (inl && !left && bl > 0) {
+        q1 = q2 = row + bl - 1;
+    } else {
+        const int a = left ? bl + 1 : bl;
+        const int lev1 = 31 - __clz(c - 1);
+        q1 = st + (int64_t)lev1 * nb + a;
+        q2 = q1 + (c - 1 - (1 << lev1));
+    }
+    const Pair u = *q1, v = *q2, ed = load_pair<NT_PAIRS>(left ? e.suf_b + l : e.pre_b + r);
+    const Pair s2 = (v.v < u.v) ? v : u;
+    if (left && s2.i <= r) return (ed.v <= s2.v) ? ed : s2;   // the left part's indices are smaller: it wins ties
+    if (!left && s2.i >= l) return (ed.v < s2.v) ? ed : s2;   // the right part's are larger: strictly smaller wins
+    // phase 3
+    const Pair other = load_pair<NT_PAIRS>(left ? e.pre_b + r : e.suf_b + l);
+    Pair best = left ? ed : other;
+    if (c > 2) {
+        const Pair *m1, *m2;
+        if (inl && a2 - bl >= 2) {
+            m1 = row + bl + 1;
+            m2 = row + a2 - 1;
+        } else {
+            const int lev2 = 31 - __clz(c - 2);
+            m1 = st + (int64_t)lev2 * nb + bl + 1;
+            m2 = m1 + (c - 2 - (1 << lev2));
+        }
+        const Pair g = *m1, h = *m2;
+        const Pair mid = (h.v < g.v) ? h : g;
+        if (mid.v < best.v) best = mid;
+    }
+    const Pair rp = left ? other : ed;
+    if (rp.v < best.v) best = rp;
     return best;
 }
 
@@ -503,23 +577,22 @@ __device__ __forceinline__ Pair answer_in_block(const EdgeTables &e, const Block
 // read, the minimum of a window of whole blocks covering bl..br is the answer
 // when its leftmost argmin lies in [l, r] (every element of [l, r] before it
 // is strictly larger); otherwise answer_failed.
-// LINES: the table is in global memory (base 256-byte aligned), where each
-// read costs a 128-byte line.  NT_PAIRS: the per-element edge pairs are
-// loaded non-temporally (measured: uniform lengths with the table in LDS
-// 0.386 -> 0.342 ms; with the table in global memory it costs the lognormal
-// 2^14 batch 10%, so that kernel keeps cached loads).
-// TWO_STAGE: a failed superset test is retried with the failing side cut off
-// (answer_failed).  It saves DRAM lines where the table misses the caches
-// (lognormal 2^6 lengths at bs=8, a 2.4 GB table: 2.14 -> 2.02 ms) and costs
-// where it hits them (lognormal 2^14 at bs=2048, 6 MB: +3.5%), so the host
-// turns it on for large tables only (kTwoStageTableBytes).
+// LINES: the table is in global memory, where each read costs a 128-byte
+// line.  NT_PAIRS: the per-element edge pairs are loaded non-temporally
+// (measured: uniform lengths with the table in LDS 0.386 -> 0.342 ms; with
+// the table in global memory it costs the lognormal 2^14 batch 10%, so that
+// kernel keeps cached loads).
+// TWO_STAGE: the phased path for tables beyond the caches
+// (answer_two_stage), which the host picks by table size
+// (kTwoStageTableBytes).
 template <bool LINES = true, bool NT_PAIRS = false, bool TWO_STAGE = false>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
     const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
     if (bl != br) {
-        const Pair sup = superset_min<LINES, TWO_STAGE>(k.st, k.nb, k.bs, bl, br);
+        if constexpr (TWO_STAGE) return answer_two_stage<NT_PAIRS>(e, k, l, r, bl, br);
+        const Pair sup = superset_min<LINES>(k.st, k.nb, bl, br);
         if (sup.i >= l && sup.i <= r) return sup;
-        return answer_failed<NT_PAIRS, TWO_STAGE>(e, k, l, r, bl, br, sup);
+        return answer_failed<NT_PAIRS>(e, k, l, r, bl, br);
     }
     return answer_in_block<NT_PAIRS>(e, k, l, r, bl);
 }
