@@ -113,6 +113,35 @@ def test_signed_zeros_infinities_and_denormals(bs, flags, gpu):
     assert (bits == 0).any() and (bits == 0x80000000).any() and (bits == 1).any()  # +0, -0, 1e-45 all answered
 
 
+@pytest.mark.parametrize("bs", [2, 3, 8, 16])
+def test_large_table_path_with_ties_zeros_and_inf_runs(bs, gpu):
+    """The phased path of block tables beyond the caches (k_query_edges<G,
+    true>: superset test, the other side's blocks with the failing side's
+    edge pair, then both pairs and the blocks between) on an array whose
+    table is large enough to take it: values in {-0, +0, 1, 2, 3} with runs
+    of +inf, n not a multiple of bs, lengths log-uniform in [2, 2^12] and
+    ranges ending at the array's end, against brute force."""
+    n = (1 << 24) + 5
+    rng = np.random.default_rng(900 + bs)
+    pool = np.array([0.0, -0.0, 1.0, 2.0, 3.0], np.float32)
+    A = pool[rng.integers(0, pool.size, n)]
+    A[(np.arange(n) // 700) % 5 == 2] = np.inf
+    q = 1 << 20
+    l = rng.integers(0, n, q)
+    ln = np.exp2(rng.uniform(1, 12, q)).astype(np.int64)
+    r = np.minimum(l + ln - 1, n - 1)
+    r[: q // 64] = n - 1  # ranges that end at the array's end
+    l[: q // 64] = np.maximum(0, n - ln[: q // 64])
+    Q = np.stack([l, r], 1).astype(np.int32)
+    ev, ei = O.rmq_brute(A, Q)
+    opts = R.query_opts(short_max=1, group_lanes=8, long_lanes=16)
+    with R.RMQIndex(_dev(A, gpu), bs, flags=R.BUILD_EDGES) as ix:
+        assert ix.kernels(opts) == ["k_query_edges<8, true>"], ix.kernels(opts)
+        v, i = ix.query(_dev(Q, gpu), opts=opts)
+        torch.cuda.synchronize()
+        assert_exact(v.cpu().numpy(), i.cpu().numpy(), ev, ei, f"bs={bs}")
+
+
 @pytest.fixture(scope="module")
 def mixed_case():
     A = O.gen_array(200003, 11, "ties16")
