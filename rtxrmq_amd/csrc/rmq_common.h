@@ -19,7 +19,7 @@ constexpr int kWave = 64;
 constexpr int kWG = 256;                 // threads per workgroup (4 waves)
 constexpr int kWavesPerWG = kWG / kWave;
 constexpr int kLdsWaves = 16;            // waves per workgroup of the LDS-table query kernel
-constexpr int64_t kTwoStageTableBytes = int64_t(64) << 20;  // block tables beyond this retry failed tests
+constexpr int64_t kTwoStageTableBytes = int64_t(64) << 20;  // block tables beyond this take answer_two_stage
 // SGPR budget for query kernels that would otherwise allocate more than 80
 // SGPRs: on gfx950 an allocation of 96 or more cuts resident waves by 40-50%
 // (measured with SQ_WAVE_CYCLES / SQ_BUSY_CYCLES) although the compiler still
