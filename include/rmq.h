@@ -282,7 +282,9 @@ int rmq_server_submit(rmq_server_t *sv, int32_t array_id, const int32_t *h_Q, in
  * gets its status, and the ticket is forgotten when the last of them returns
  * (a later wait on it returns RMQ_ERR_INVALID_ARG). */
 int rmq_server_wait(rmq_server_t *sv, uint64_t ticket, double *latency_ms);
-/* Latency percentiles and throughput since the last reset (reset != 0 clears). */
+/* Latency percentiles and throughput since the last reset (reset != 0
+ * clears).  p50/p99/mean/max are taken over the latest 2^20 requests at most;
+ * the counts and wall_ms cover every request since the reset. */
 int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t reset);
 /* Finishes the launches in flight, then frees everything (not the caller's
  * arrays).  Requests not yet launched are dropped; no thread may submit to or

@@ -464,6 +464,10 @@ struct Slot {
 
 int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
 
+// Latency percentiles are taken over at most this many of the latest
+// requests, so a server that never resets its statistics holds 8 MiB of them.
+constexpr size_t kLatencyWindow = (size_t)1 << 20;
+
 // Device bytes a request holds while in flight: its queries and outputs,
 // each rounded to 256 bytes (the carving launch_group does).
 int64_t request_bytes(const Request &r) {
@@ -512,7 +516,8 @@ struct rmq_server {
     std::thread sched;
     std::vector<std::thread> completers;
     // statistics
-    std::vector<double> lat_ms;
+    std::vector<double> lat_ms;  // the last kLatencyWindow latencies, as a ring from lat_next
+    size_t lat_next = 0;
     int64_t requests_done = 0, queries_done = 0, launches = 0, failed = 0, batched_requests = 0;
     double first_submit = -1, last_done = -1;
     double sum_enqueue_ms = 0, sum_device_ms = 0, sum_wait_ms = 0;
@@ -709,7 +714,12 @@ void completer_loop(rmq_server *sv, size_t si) {
                 if (r.status == RMQ_OK) {
                     ++sv->requests_done;
                     sv->queries_done += r.q;
-                    sv->lat_ms.push_back(t - r.t_submit);
+                    if (sv->lat_ms.size() < kLatencyWindow) {
+                        sv->lat_ms.push_back(t - r.t_submit);
+                    } else {
+                        sv->lat_ms[sv->lat_next] = t - r.t_submit;
+                        sv->lat_next = (sv->lat_next + 1) % kLatencyWindow;
+                    }
                 } else {
                     ++sv->failed;
                 }
@@ -948,6 +958,7 @@ RMQ_API int rmq_server_stats(rmq_server_t *sv, rmq_server_stats_t *st, int32_t r
     }
     if (reset) {
         sv->lat_ms.clear();
+        sv->lat_next = 0;
         sv->requests_done = sv->queries_done = sv->launches = sv->failed = sv->batched_requests = 0;
         sv->first_submit = sv->last_done = -1;
         sv->peak_inflight = sv->inflight;

@@ -215,13 +215,13 @@ __global__ __launch_bounds__(kWG) void k_block_sub_scan(const Pair *__restrict__
 // Step 3: one thread per element: block-level prefix = (sub-blocks before,
 // prefix inside its sub-block); block-level suffix = (suffix inside its
 // sub-block, sub-blocks after).  The earlier part wins ties.
-__global__ void k_block_edges(int n, int bs, uint64_t magic, int nsub, const Pair *__restrict__ pre_s,
+__global__ void k_block_edges(int n, int bs, BlockDiv div, int nsub, const Pair *__restrict__ pre_s,
                               const Pair *__restrict__ suf_s, const Pair *__restrict__ excl_pre,
                               const Pair *__restrict__ excl_suf, Pair *__restrict__ pre_b, Pair *__restrict__ suf_b) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = (int)t;
-    const int b = block_of(i, magic);
+    const int b = block_of(i, div);
     const int64_t g = (int64_t)b * nsub + ((i - b * bs) >> kSubShift);
     Pair p = excl_pre[g];
     const Pair x = pre_s[i];
@@ -714,7 +714,7 @@ int build_edges(rmq_handle *h, const HandleSizes &z, hipStream_t s) {
                                                                                            excl_pre, excl_suf);
         rc = hip_rc(hipGetLastError());
         if (rc == RMQ_OK) {
-            k_block_edges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, bs, block_magic(bs), nsub, pre_s, suf_s,
+            k_block_edges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((int)n, bs, block_div(bs), nsub, pre_s, suf_s,
                                                                       excl_pre, excl_suf, pre_b, suf_b);
             rc = hip_rc(hipGetLastError());
         }

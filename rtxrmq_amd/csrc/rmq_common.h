@@ -219,22 +219,34 @@ __device__ __forceinline__ TileMasks load_tile(const int2 *__restrict__ Q, int64
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short), is_short, ok && !is_short};
 }
 
-// Block index of element e: e / bs by one multiply-high with the handle's
-// magic = floor((2^64-1)/bs) + 1 (exact for 0 <= e < 2^31 and bs >= 2);
-// magic 0 means bs == 1.  Every caller passes an element index (e >= 0), so
-// e is zero-extended: a 32 x 64-bit product, two VALU multiplies where the
-// sign-extended form takes about nine.
-__host__ __device__ __forceinline__ uint64_t block_magic(int bs) { return bs > 1 ? UINT64_MAX / (uint64_t)bs + 1 : 0; }
+// Block index of element e (0 <= e < 2^31): e / bs as mul_hi(2e, magic) >>
+// shift, one 32-bit multiply, with shift = ceil(log2 bs) and magic =
+// ceil(2^(31+shift) / bs), which is below 2^32 for every bs < 2^31.  Exact:
+// magic = 2^(31+shift)/bs + d with 0 <= d < 1, so e*magic / 2^(31+shift) =
+// e/bs + e*d / 2^(31+shift), and the error e*d / 2^(31+shift) < 2^-shift <=
+// 1/bs never reaches the next multiple of 1/bs above e/bs.  bs = 1 gives
+// magic = 2^31, shift 0: the identity.
+struct BlockDiv {
+    uint32_t magic;
+    int32_t shift;
+};
 
-__device__ __forceinline__ int block_of(int e, uint64_t magic) {
-    return magic ? (int)__umul64hi((uint64_t)(uint32_t)e, magic) : e;
+__host__ __device__ __forceinline__ BlockDiv block_div(int bs) {
+    int sh = 0;
+    while ((int64_t)1 << sh < (int64_t)bs) ++sh;
+    return BlockDiv{(uint32_t)((((uint64_t)1 << (31 + sh)) + (uint64_t)bs - 1) / (uint64_t)bs), sh};
+}
+
+__device__ __forceinline__ int block_of(int e, BlockDiv d) {
+    return (int)(__umulhi((uint32_t)e << 1, d.magic) >> d.shift);
 }
 
 // Leftmost minimum of the c >= 1 entries [a, a+c) of a sparse table whose
-// rows hold `stride` entries; the left window wins ties.
-__device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, int64_t stride, int64_t a, int c) {
+// rows hold `stride` entries; the left window wins ties.  Row offsets are
+// zero-extended 32 x 32-bit products (one multiply-add).
+__device__ __forceinline__ Pair table_min(const Pair *__restrict__ st, uint32_t stride, int64_t a, int c) {
     const int lev = 31 - __clz(c);
-    const Pair *row = st + (int64_t)lev * stride;
+    const Pair *row = st + (uint64_t)(uint32_t)lev * stride;
     const Pair x = row[a];
     const Pair y = row[a + c - (1 << lev)];
     return (y.v < x.v) ? y : x;
@@ -246,7 +258,7 @@ struct Blocks {
     const Pair *st;   // levels rows of nb pairs; row 0 = each block's (min, leftmost argmin)
     const Pair *sub;  // sparse table over the nb * nsub sub-block minima (row 0: each 32-element sub-block's (min,
                       // leftmost argmin)); null when bs <= 32
-    uint64_t magic;   // block_magic(bs)
+    BlockDiv div;     // block_div(bs)
     int n, bs, nb, nsub;
 };
 
@@ -272,9 +284,9 @@ __device__ __forceinline__ TileMasks classify_edges(int2 p, int64_t q, const Blo
     if (WRITE_INVALID && in && !ok) write_result<NT>(out_v, out_i, qi, __int_as_float(0x7FC00000), -1);
     bool is_short = ok && (r - l) < short_max;
     if (ok && !is_short && r - l < kSub) {
-        const int bl = block_of(l, k.magic);
+        const int bl = block_of(l, k.div);
         const int base = bl * k.bs;  // <= l
-        is_short = block_of(r, k.magic) == bl && ((l - base) >> kSubShift) == ((r - base) >> kSubShift);
+        is_short = block_of(r, k.div) == bl && ((l - base) >> kSubShift) == ((r - base) >> kSubShift);
     }
     return TileMasks{l, r, __ballot(is_short), __ballot(ok && !is_short), is_short, ok && !is_short};
 }
@@ -318,7 +330,7 @@ __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi
             const int64_t row = (int64_t)b * k.nsub;
             scan_range<G, U>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
             if (whole > kHierTable) {
-                const Pair m = table_min(k.sub, (int64_t)k.nb * k.nsub, row + jl + 1, whole);
+                const Pair m = table_min(k.sub, (uint32_t)(k.nb * k.nsub), row + jl + 1, whole);
                 take_if_better(m.v, m.i, bv, bi);
             } else {
                 scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
@@ -344,7 +356,7 @@ __device__ __forceinline__ void answer_partial(const Blocks &k, int l, int r, in
     if constexpr (!LONG) {
         scan_range<G, U>(k.A, l, r, gl, bv, bi);
     } else {
-        const int lB = block_of(l, k.magic), rB = block_of(r, k.magic);
+        const int lB = block_of(l, k.div), rB = block_of(r, k.div);
         if (lB == rB) {
             scan_hier<G, U>(k, lB, l, r, gl, bv, bi);
         } else {
@@ -555,13 +567,13 @@ template <bool NT_PAIRS>
 __device__ __forceinline__ Pair answer_in_block(const EdgeTables &e, const Blocks &k, int l, int r, int bl) {
     const int base = bl * k.bs;
     const int jl = (l - base) >> kSubShift, jr = (r - base) >> kSubShift;
-    const Pair sup = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl, jr - jl + 1);
+    const Pair sup = table_min(e.st_s, (uint32_t)e.subs_total, (int64_t)bl * e.nsub + jl, jr - jl + 1);
     if (sup.i >= l && sup.i <= r) return sup;
     Pair best, right;
     if constexpr (NT_PAIRS) best = nt_load_pair(e.suf_s + l); else best = e.suf_s[l];
     if constexpr (NT_PAIRS) right = nt_load_pair(e.pre_s + r); else right = e.pre_s[r];
     if (jr - jl > 1) {
-        const Pair mid = table_min(e.st_s, e.subs_total, (int64_t)bl * e.nsub + jl + 1, jr - jl - 1);
+        const Pair mid = table_min(e.st_s, (uint32_t)e.subs_total, (int64_t)bl * e.nsub + jl + 1, jr - jl - 1);
         if (mid.v < best.v) best = mid;
     }
     if (right.v < best.v) best = right;
@@ -587,7 +599,7 @@ __device__ __forceinline__ Pair answer_in_block(const EdgeTables &e, const Block
 // (kTwoStageTableBytes).
 template <bool LINES = true, bool NT_PAIRS = false, bool TWO_STAGE = false>
 __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &k, int l, int r) {
-    const int bl = block_of(l, k.magic), br = block_of(r, k.magic);
+    const int bl = block_of(l, k.div), br = block_of(r, k.div);
     if (bl != br) {
         if constexpr (TWO_STAGE) return answer_two_stage<NT_PAIRS>(e, k, l, r, bl, br);
         const Pair sup = superset_min<LINES>(k.st, k.nb, bl, br);
@@ -629,7 +641,7 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
         // table reads; only the others go through the G-lane path.
         bool done = false;
         if ((mk >> lane) & 1) {
-            const int lB = block_of(m.l, k.magic), rB = block_of(m.r, k.magic);
+            const int lB = block_of(m.l, k.div), rB = block_of(m.r, k.div);
             if (lB != rB) {
                 const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
                 if (sup.i >= m.l && sup.i <= m.r) {
@@ -790,7 +802,7 @@ struct rmq_handle {
 namespace rmqx {
 
 inline Blocks blocks_of(const rmq_handle *h) {
-    return Blocks{h->A, h->st, h->sub, block_magic(h->bs), (int)h->n, h->bs, (int)h->nb, h->nsub};
+    return Blocks{h->A, h->st, h->sub, block_div(h->bs), (int)h->n, h->bs, (int)h->nb, h->nsub};
 }
 
 inline int check_device(const rmq_handle *h) {
