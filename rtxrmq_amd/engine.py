@@ -488,6 +488,8 @@ class Server(_Native):
         return rc, lat.value
 
     def stats(self, reset: bool = False) -> dict:
+        """Counters since the last reset; latency percentiles over the latest
+        2^20 requests of them at most (rmq_server_stats)."""
         st = N.ServerStats()
         N.check(N.lib().rmq_server_stats(self._h, ctypes.byref(st), 1 if reset else 0), "rmq_server_stats")
         return {name: getattr(st, name) for name, _ in N.ServerStats._fields_}
