@@ -36,7 +36,7 @@ over bs = 2^10 .. 2^18, answers compared.
 cpu_baseline: the reference's HRMQ (RMQRMM64, built from the reference's own
 sources into oracle/_ref by oracle/Makefile) on this host's cores, threads
 over queries as src/cpu_methods.h:104-110, on a bounded sample of each batch,
-its answers checked against the GPU's; this build's own HRMQ timed on the
+its answers checked against the GPU's; the engine's own HRMQ (rmq_hrmq_*) timed on the
 same sample; CPU BASE (algs 0/100) and HRMQ at n=2^20, q=2^20 beside the GPU.
 """
 from __future__ import annotations
@@ -563,7 +563,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
     t0 = time.perf_counter()
     port = R.HostHRMQ(A_host)
     port_build_s = time.perf_counter() - t0
-    log(f"cpu_baseline: reference built in {build_s:.2f} s, this build's HRMQ in {port_build_s:.2f} s; "
+    log(f"cpu_baseline: reference built in {build_s:.2f} s, the engine's HRMQ in {port_build_s:.2f} s; "
         f"timing queries on {nt} threads")
     total_q, total_t, port_t, per = 0, 0.0, 0.0, {}
     for lr, Q in queries.items():
@@ -584,7 +584,7 @@ def cpu_baseline(A_host: np.ndarray, queries: dict, target_s: float, gpu_answers
         port_bad = int(((pi != hi) | (pv.view(np.int32) != hv.view(np.int32))).sum())
         per[DIST_NAMES[lr]] = {"rmq_per_s": m / dt, "queries": m, "port_rmq_per_s": m / pdt,
                                "gpu_vs_hrmq": cmp, "port_vs_reference_mismatches": port_bad}
-        log(f"cpu_baseline: {DIST_NAMES[lr]} reference {m / dt:.3e} RMQ/s, this build's {m / pdt:.3e} RMQ/s over "
+        log(f"cpu_baseline: {DIST_NAMES[lr]} reference {m / dt:.3e} RMQ/s, the engine's {m / pdt:.3e} RMQ/s over "
             f"{m} queries; GPU vs HRMQ: {cmp}; port vs reference mismatches {port_bad}")
         if cmp["value_mismatches"] or cmp["index_mismatches"] or port_bad:
             raise SystemExit(f"cpu_baseline: answers differ on {DIST_NAMES[lr]}")
