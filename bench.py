@@ -170,8 +170,21 @@ def left_part_min(A, sub, bs: int, l):
     return v
 
 
+MID_MAX_BLOCKS = 3072  # coarse blocks of an RMQ_BUILD_MID handle at most (kMidMaxBlocks)
+
+
+def mid_shift_of(nb: int) -> int:
+    """log2 of the blocks per coarse block of an RMQ_BUILD_MID handle with nb
+    blocks: the smallest that leaves at most MID_MAX_BLOCKS coarse blocks."""
+    s = 0
+    while ((nb - 1) >> s) + 1 > MID_MAX_BLOCKS:
+        s += 1
+    return s
+
+
 def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, edges: bool = False,
-                      sub: BlockLevel | None = None, A=None, lines: bool = False, two_stage: bool = False) -> dict:
+                      sub: BlockLevel | None = None, A=None, lines: bool = False, two_stage: bool = False,
+                      mid_shift: int | None = None) -> dict:
     """Per pass: bytes in the long and the short class, their query counts,
     the fraction of cross-block edge parts computed, the random gathers of
     per-element edge pairs (with edge tables), the sparse-table entries read
@@ -203,7 +216,11 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     lines: the superset test reads one level-up entry instead of two entries
     in different 128-byte lines (BlockLevel.superset), as the edge-table
     kernel with its table in global memory does; with two_stage as well, an
-    argmin of that entry past block rB is followed by the two entries."""
+    argmin of that entry past block rB is followed by the two entries.
+    mid_shift (edge tables with coarse rows, RMQ_BUILD_MID): a range across
+    blocks whose blocks lie in at most 3 coarse blocks of 2^mid_shift blocks
+    first takes the minimum of those coarse blocks from LDS; when its argmin
+    lies in [l, r] that is the answer, B = 8 + out, with no table read."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
@@ -235,6 +252,18 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             long_b += float((8 + scan_bytes(l[same], r[same], bs) + out_bytes).sum())
         x = ~same
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
+        if edges and mid_shift is not None:
+            cl, cr = lB >> mid_shift, rB >> mid_shift
+            near = cr - cl < 3  # at most 3 coarse blocks: the two LDS rows
+            a0 = cl << mid_shift
+            end = torch.clamp((cr + 1) << mid_shift, max=lvl.nb)
+            ci = lvl.range_min(a0[near], (end - a0)[near])[1]
+            hit = torch.zeros_like(near)
+            hit[near] = (ci >= l[near]) & (ci <= r[near])
+            long_b += float(8 * int(hit.sum()) + out_bytes * int(hit.sum()))
+            parts += 2 * int(hit.sum())
+            x = ~hit
+            l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         _, sup_i, sr = lvl.superset(lB, rB, edges and lines, retry=edges and lines and two_stage)
         fast = (sup_i >= l) & (sup_i <= r)
         long_b += float((8 + 8 * sr[fast] + out_bytes).sum())
@@ -373,8 +402,9 @@ def workload_key(n: int, q: int, bs_for: dict, opts_for: dict, edges_for: dict |
     parts = []
     for lr in DISTS:
         o = opts_for[lr]
-        parts.append(f"{lr}:bs={bs_for[lr]}{'+edges' if edges_for and edges_for[lr] else ''}"
-                     f"/s{o.short_max}/g{o.group_lanes}/l{o.long_lanes}")
+        fl = int(edges_for[lr]) if edges_for else 0
+        tag = ("+edges" if fl & 1 else "") + ("+mid" if fl & 2 else "")
+        parts.append(f"{lr}:bs={bs_for[lr]}{tag}/s{o.short_max}/g{o.group_lanes}/l{o.long_lanes}")
     return f"n={n} q={q} value+index " + " ".join(parts)
 
 
@@ -686,9 +716,10 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
             raise ValueError("the byte model needs block sizes of at most 32 or multiples of 32 with edge tables")
         kname = kernels_for[lr][0] if kernels_for else ""
         lines = bool(e and not kname.startswith("k_query_edges_lds"))
-        two = bool(e and kname.startswith("k_query_edges<") and kname.endswith(", true>"))
+        two = bool(e and kname.startswith(("k_query_edges<", "k_query_edges_mid<")) and kname.endswith(", true>"))
+        mid = mid_shift_of(levels[bs_for[lr]].nb) if e and kname.startswith("k_query_edges_mid<") else None
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
-                               sub=levels.get(32) if e else None, A=A, lines=lines, two_stage=two)
+                               sub=levels.get(32) if e else None, A=A, lines=lines, two_stage=two, mid_shift=mid)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         lit = literal_bytes(Q_dev[lr], bs_for[lr])
         ms = [s[lr]["total_ms"] for s in steps_out]
@@ -743,6 +774,8 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
     for k, v in kernels.items():
         if k.startswith("k_query_edges_lds"):
             per_kernel[k]["table_in_lds"] = True
+        if k.startswith("k_query_edges_mid"):
+            per_kernel[k]["coarse_rows_in_lds"] = True
         if v[3] or v[4]:
             per_kernel[k]["edge_gathers_per_launch"] = v[3] / v[2]
             per_kernel[k]["table_reads_per_launch"] = v[4] / v[2]
@@ -790,8 +823,10 @@ def run_batched(R, torch, dev, args) -> dict:
                         "rmq_gen_queries")
     torch.cuda.synchronize()
     big = int(np.argmax(ns))
+    # the plan's kernel reads no coarse rows: tuned among plain and edge-table handles
     pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096],
-                  cache_path=args.tune_cache or None, refresh=args.retune, want_value=False)
+                  cache_path=args.tune_cache or None, refresh=args.retune, want_value=False,
+                  flags_allowed=R.BUILD_EDGES)
     opts, bs, flags = pick["opts"], pick["bs"], pick["flags"]
     log(f"batched: tuned on the largest array (n={int(ns[big])}): bs={bs} edges={bool(flags)} "
         f"short_max={opts.short_max} group_lanes={opts.group_lanes} long_lanes={opts.long_lanes}")
@@ -860,7 +895,8 @@ def run_batched_classes(R, torch, args, ns, A_views, Q_views, q_off, out_ref, ti
         members = np.nonzero(cls == c)[0]
         rep = int(members[np.argmax(ns[members])])
         picks[c] = R.tune(A_views[rep], Q_views[rep], bs_list=[16, 32, 64, 128, 256, 512, 1024, 4096],
-                          cache_path=args.tune_cache or None, refresh=args.retune, want_value=False)
+                          cache_path=args.tune_cache or None, refresh=args.retune, want_value=False,
+                          flags_allowed=R.BUILD_EDGES)
     q_per = {c: int(ns[cls == c].sum()) for c in picks}
     top = max(q_per, key=q_per.get)
     opts = picks[top]["opts"]
@@ -916,8 +952,9 @@ def run_served(R, torch, dev, args) -> dict:
     arrays = [R.gen_array(int(n), args.seed * 31 + i, "uniform", device=dev) for i, n in enumerate(ns)]
     big = int(np.argmax(ns))
     sample = torch.cat([R.gen_queries_device(1 << 20, int(ns[big]), lr, 77, device=dev) for lr in (-1, -7, -8)])
+    # the server's kernel reads no coarse rows: tuned among plain and edge-table handles
     pick = R.tune(arrays[big], sample, bs_list=[16, 32, 64, 128, 256, 1024], cache_path=args.tune_cache or None,
-                  refresh=args.retune, want_value=False)
+                  refresh=args.retune, want_value=False, flags_allowed=R.BUILD_EDGES)
     bs, opts, flags = pick["bs"], pick["opts"], pick["flags"]
     log(f"served: bs={bs} edges={bool(flags)} short_max={opts.short_max} group_lanes={opts.group_lanes} "
         f"long_lanes={opts.long_lanes}")
@@ -1256,7 +1293,8 @@ def main() -> int:
         t0 = time.perf_counter()
         picks[lr] = R.tune(A, sample, cache_path=args.tune_cache or None, refresh=args.retune)
         p = picks[lr]
-        log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} edges={bool(p['flags'])} short_max={p['opts'].short_max} "
+        log(f"bench: autotune {DIST_NAMES[lr]}: bs={p['bs']} edges={bool(p['flags'])} "
+            f"mid={bool(p['flags'] & R.BUILD_MID)} short_max={p['opts'].short_max} "
             f"group_lanes={p['opts'].group_lanes} long_lanes={p['opts'].long_lanes} "
             f"({'cached' if p['from_cache'] else str(p['n_tried']) + ' configs'}, {time.perf_counter() - t0:.1f} s, "
             f"key {p['key']})")
@@ -1267,7 +1305,8 @@ def main() -> int:
             t0 = time.perf_counter()
             indexes[key] = R.RMQIndex(A, key[0], flags=key[1])
             torch.cuda.synchronize()
-            build_ms[f"bs={key[0]}{' edges' if key[1] else ''}"] = (time.perf_counter() - t0) * 1e3
+            build_ms[f"bs={key[0]}{' edges' if key[1] else ''}{' mid' if key[1] & R.BUILD_MID else ''}"] = \
+                (time.perf_counter() - t0) * 1e3
     index_for = {lr: indexes[(picks[lr]["bs"], picks[lr]["flags"])] for lr in DISTS}
     opts_for = {lr: picks[lr]["opts"] for lr in DISTS}
     bs_for = {lr: picks[lr]["bs"] for lr in DISTS}
@@ -1393,6 +1432,7 @@ def main() -> int:
                    "n": n, "q": q, "dists": [DIST_NAMES[lr] for lr in DISTS],
                    "bs": {DIST_NAMES[lr]: bs_for[lr] for lr in DISTS},
                    "edge_tables": {DIST_NAMES[lr]: bool(picks[lr]["flags"]) for lr in DISTS},
+                   "coarse_rows": {DIST_NAMES[lr]: bool(picks[lr]["flags"] & R.BUILD_MID) for lr in DISTS},
                    "construction_ms": {str(k): v for k, v in build_ms.items()},
                    "tune_sample": min(q, args.tune_sample),
                    "tune_source": {DIST_NAMES[lr]: ("cache" if picks[lr]["from_cache"] else "timed") for lr in DISTS}},

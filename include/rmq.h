@@ -50,6 +50,7 @@ typedef struct rmq_handle rmq_handle_t;
  * stream so that host and device produce the same array). */
 /* rmq_build flags */
 #define RMQ_BUILD_EDGES 1u /* precompute the edge-block searches (32 B per element): see rmq_build */
+#define RMQ_BUILD_MID 2u   /* with RMQ_BUILD_EDGES: coarse rows for an LDS superset test (48 KB at most) */
 
 #define RMQ_GEN_UNIFORM 0 /* k * 2^-24, k uniform in [0, 2^24): U[0,1) exactly      */
 #define RMQ_GEN_TIES16 1  /* integers 0..15 as float (heavy ties)                  */
@@ -87,7 +88,14 @@ typedef struct rmq_query_times {
  *      the same inside 32-element sub-blocks with a sparse table over them
  *      (32 bytes per element): every range not inside one 32-element
  *      sub-block (one block when bs <= 32) is then answered by one lane with
- *      a few O(1) lookups instead of edge scans.
+ *      a few O(1) lookups instead of edge scans.  RMQ_BUILD_MID (only with
+ *      RMQ_BUILD_EDGES) also keeps two rows of a sparse table over coarse
+ *      blocks of 2^j blocks, the fewest that make at most 3072 of them; a
+ *      query pass whose block table does not fit in LDS copies those rows
+ *      into LDS and answers a range from them when the leftmost minimum of
+ *      its (at most 3) coarse blocks lies inside it, before any table read
+ *      in global memory.  It pays for ranges a sizeable fraction of a coarse
+ *      block long (the autotuner decides).
  *      Stream-ordered; the handle is usable on stream s at once, on other
  *      streams after s is synchronised. ---- */
 int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out);
@@ -146,7 +154,7 @@ int rmq_query_opts_default(const rmq_handle_t *h, rmq_query_opts_t *opts);
  * distribution).  key is the distribution signature the cache is keyed by. */
 typedef struct rmq_tune_result {
     int32_t bs;
-    uint32_t build_flags; /* 0 or RMQ_BUILD_EDGES */
+    uint32_t build_flags; /* 0, RMQ_BUILD_EDGES, or RMQ_BUILD_EDGES | RMQ_BUILD_MID */
     rmq_query_opts_t opts;
     float best_ms;      /* one query pass over the tuning sample */
     int32_t n_tried;    /* configurations timed (0 when taken from the cache) */
@@ -157,7 +165,9 @@ typedef struct rmq_tune_result {
 /* ---- autotuner (no counterpart in the reference, whose sweeps are shell
  *      scripts over --nb, scripts/perf-fulltest.sh:45-58): times query passes
  *      over the sample d_Q[0..q) for each block size in bs_list (NULL/0: 2^3 ..
- *      2^18), with and without edge tables (RMQ_BUILD_EDGES), then, for the
+ *      2^18), with and without edge tables (RMQ_BUILD_EDGES), and with edge
+ *      tables and coarse rows (RMQ_BUILD_MID) where the block table does not
+ *      fit in LDS, then, for the
  *      four fastest of those, the short-range threshold and lanes per query,
  *      and returns the fastest configuration (build it with rmq_build(..., bs,
  *      build_flags, ...)).  Results are cached in the text file cache_path
@@ -170,6 +180,14 @@ typedef struct rmq_tune_result {
  *      ask for).  Synchronising; builds and frees its own handles. ---- */
 int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,
              float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s, rmq_tune_result_t *out);
+/* rmq_tune choosing only among build flags within flags_allowed (0: plain
+ * handles only; RMQ_BUILD_EDGES: with or without edge tables; RMQ_BUILD_EDGES
+ * | RMQ_BUILD_MID: everything, as rmq_tune).  For handles that go into plans
+ * or a server, whose kernel reads no coarse rows, pass RMQ_BUILD_EDGES.  A
+ * restricted set is part of the cache key ("/f<flags_allowed>"). */
+int rmq_tune_ex(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list, int32_t n_bs,
+                uint32_t flags_allowed, float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s,
+                rmq_tune_result_t *out);
 
 /* ---- multi-array plans: many arrays (the batched mode) or many requests (the
  *      serving loop) answered by ONE persistent launch.  No counterpart in the
@@ -182,7 +200,9 @@ int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const i
  *      may be NULL as a whole).  opts applies to every array (NULL =
  *      defaults).  The pointers are captured at creation: the buffers must
  *      outlive the plan, their contents may change between launches.
- *      Creation is synchronising; launches are stream-ordered. ---- */
+ *      Creation is synchronising; launches are stream-ordered.  The plan's
+ *      kernel reads each handle's block table and edge tables; coarse rows
+ *      (RMQ_BUILD_MID) are not used. ---- */
 typedef struct rmq_plan rmq_plan_t;
 int rmq_plan_create(const rmq_handle_t *const *handles, int32_t m, const int32_t *const *d_Qs, const int64_t *qs,
                     float *const *d_vals, int32_t *const *d_idxs, const rmq_query_opts_t *opts, int32_t item_queries,

@@ -25,6 +25,7 @@ RMQ_ERR_LIMIT = -6
 
 GEN_KINDS = {"uniform": 0, "ties16": 1, "desc": 2, "const": 3}
 BUILD_EDGES = 1  # RMQ_BUILD_EDGES
+BUILD_MID = 2  # RMQ_BUILD_MID (with BUILD_EDGES)
 SERVER_STAGE_COPIES = 1  # RMQ_SERVER_STAGE_COPIES
 
 _vp = ctypes.c_void_p
@@ -97,6 +98,8 @@ def _declare(lib):
         "rmq_hrmq_free": (None, [_vp]),
         "rmq_tune": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, ctypes.c_char_p, _vp,
                                     ctypes.POINTER(TuneResult)]),
+        "rmq_tune_ex": (ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _u32, _vp, _vp, ctypes.c_char_p, _vp,
+                                       ctypes.POINTER(TuneResult)]),
         "rmq_gen_array": (ctypes.c_int, [_vp, _i64, _u64, _i32, _vp]),
         "rmq_gen_array_host": (ctypes.c_int, [_vp, _i64, _u64, _i32]),
         "rmq_gen_queries_host": (ctypes.c_int, [_vp, _i64, _i32, _i32, _u64, _i32]),

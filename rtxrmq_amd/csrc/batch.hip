@@ -833,7 +833,7 @@ RMQ_API int rmq_server_create(const rmq_server_config_t *cfg, rmq_server_t **out
 
 RMQ_API int rmq_server_add_array_ex(rmq_server_t *sv, const float *d_A, int64_t n, int32_t bs, uint32_t flags,
                                     int32_t *array_id) {
-    if (!sv || !array_id || !d_A || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES))
+    if (!sv || !array_id || !d_A || n <= 0 || n > INT32_MAX || bs <= 0 || !build_flags_ok(flags))
         return RMQ_ERR_INVALID_ARG;
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return RMQ_ERR_HIP;

@@ -58,6 +58,7 @@ struct Args {
     int array_kind = RMQ_GEN_UNIFORM;
     bool qgen_ref = false;  // the reference's own query stream (src/rand.cuh:116-201)
     bool edges = false;     // build edge tables (RMQ_BUILD_EDGES)
+    bool mid = false;       // with edge tables, coarse rows for an LDS superset test (RMQ_BUILD_MID)
     bool tune = false;      // pick bs, edge tables and thresholds with rmq_tune
     bool bs_given = false;  // --bs or --nb on the command line: --tune keeps that block size
     std::string tune_cache;
@@ -89,8 +90,9 @@ void usage() {
             "  --group-lanes <g>     lanes per short query: 4, 8, 16 or 64\n"
             "  --long-lanes <g>      lanes per long query: 4, 8, 16 or 64\n"
             "  --edges               build edge tables: O(1) answers for ranges not inside a 32-element sub-block\n"
+            "  --mid                 with --edges: coarse rows in LDS tried before the block table\n"
             "  --tune[=cache]        autotune bs, edge tables and thresholds on the first 2^24 queries\n"
-            "                        (overrides --edges and the lane options; with --bs/--nb the block size\n"
+            "                        (overrides --edges/--mid and the lane options; with --bs/--nb the block size\n"
             "                        stays and only edge tables and thresholds are tuned; optional pick cache)\n"
             "  --array <kind>        uniform | ties16 | desc | const (default uniform)\n"
             "  --qgen <gen>          ours (default: counter-based, thread-count independent) |\n"
@@ -121,7 +123,7 @@ bool parse(int argc, char **argv, Args &a) {
     a.nb = a.n / a.bs;
     a.seed = (int)time(nullptr);
     enum { O_BS = 1, O_NB, O_REPS, O_DEV, O_NT, O_SEED, O_CHECK, O_TIME, O_POWER, O_SHORT, O_GROUP, O_ARRAY, O_LONG, O_QGEN,
-           O_EDGES, O_TUNE };
+           O_EDGES, O_TUNE, O_MID };
     static struct option opts[] = {
         {"bs", required_argument, nullptr, O_BS},         {"nb", required_argument, nullptr, O_NB},
         {"reps", required_argument, nullptr, O_REPS},     {"dev", required_argument, nullptr, O_DEV},
@@ -131,7 +133,8 @@ bool parse(int argc, char **argv, Args &a) {
         {"short-max", required_argument, nullptr, O_SHORT}, {"group-lanes", required_argument, nullptr, O_GROUP},
         {"array", required_argument, nullptr, O_ARRAY},   {"long-lanes", required_argument, nullptr, O_LONG},
         {"qgen", required_argument, nullptr, O_QGEN},     {"edges", no_argument, nullptr, O_EDGES},
-        {"tune", optional_argument, nullptr, O_TUNE},     {nullptr, 0, nullptr, 0}};
+        {"tune", optional_argument, nullptr, O_TUNE},     {"mid", no_argument, nullptr, O_MID},
+        {nullptr, 0, nullptr, 0}};
     int opt, idx;
     optind = 5;
     while ((opt = getopt_long(argc, argv, "", opts, &idx)) != -1) {
@@ -149,6 +152,7 @@ bool parse(int argc, char **argv, Args &a) {
         case O_GROUP: a.group_lanes = atoi(optarg); break;
         case O_LONG: a.long_lanes = atoi(optarg); break;
         case O_EDGES: a.edges = true; break;
+        case O_MID: a.mid = true; break;
         case O_TUNE: a.tune = true; if (optarg) a.tune_cache = optarg; break;
         case O_QGEN:
             if (!strcmp(optarg, "ref")) a.qgen_ref = true;
@@ -347,10 +351,11 @@ int main(int argc, char **argv) {
         a.bs = pick.bs;
         a.nb = a.n / a.bs;
         a.edges = (pick.build_flags & RMQ_BUILD_EDGES) != 0;
+        a.mid = (pick.build_flags & RMQ_BUILD_MID) != 0;
         opts = pick.opts;
-        printf("done: %f secs [bs=%d edges=%d short_max=%d group_lanes=%d long_lanes=%d; %s]\n",
-               (now_ms() - t0) / 1000.0, a.bs, (int)a.edges, opts.short_max, opts.group_lanes, opts.long_lanes,
-               pick.from_cache ? "cached" : "timed");
+        printf("done: %f secs [bs=%d edges=%d mid=%d short_max=%d group_lanes=%d long_lanes=%d; %s]\n",
+               (now_ms() - t0) / 1000.0, a.bs, (int)a.edges, (int)a.mid, opts.short_max, opts.group_lanes,
+               opts.long_lanes, pick.from_cache ? "cached" : "timed");
     }
 
     csv_prefix(a);
@@ -406,10 +411,12 @@ int main(int argc, char **argv) {
         else hip_ok(hipMalloc(&d_val, sizeof(float) * (size_t)a.q), "hipMalloc(out)");
         rmq_handle_t *h = nullptr;
         if (engine) {
-            printf("Building block index (bs=%i%s)........", a.bs, a.edges ? ", edge tables" : "");
+            printf("Building block index (bs=%i%s%s)........", a.bs, a.edges ? ", edge tables" : "",
+                   a.edges && a.mid ? ", coarse rows" : "");
             fflush(stdout);
             t0 = now_ms();
-            if (int rc = rmq_build(dA, a.n, a.bs, a.edges ? RMQ_BUILD_EDGES : 0u, nullptr, &h)) die("rmq_build", rc);
+            const uint32_t flags = a.edges ? (RMQ_BUILD_EDGES | (a.mid ? RMQ_BUILD_MID : 0u)) : 0u;
+            if (int rc = rmq_build(dA, a.n, a.bs, flags, nullptr, &h)) die("rmq_build", rc);
             hip_ok(hipDeviceSynchronize(), "build");
             construction_ms = now_ms() - t0;
             int64_t nb = 0, bytes = 0;

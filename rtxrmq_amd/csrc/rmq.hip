@@ -259,6 +259,50 @@ __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, E
     }
 }
 
+// k_query_edges for a handle with coarse rows (RMQ_BUILD_MID): workgroups of
+// kLdsWaves waves, two per CU, first copy the coarse rows into LDS; every
+// long range then tries the coarse superset test (answer_edges_mid) before
+// the global table.
+template <int G, bool TWO>
+__global__ __launch_bounds__(kLdsWaves *kWave) RMQ_SGPR_BUDGET void k_query_edges_mid(
+    Blocks k, EdgeTables e, const Pair *__restrict__ mid_g, int mid_shift, int mid_nb, const int2 *__restrict__ Q,
+    int64_t q, int short_max, float *__restrict__ out_v, int32_t *__restrict__ out_i) {
+    __shared__ Pair mid[kMidLevels * kMidMaxBlocks];
+    __shared__ TileStage stage[kLdsWaves];
+    for (int j = threadIdx.x; j < kMidLevels * mid_nb; j += kLdsWaves * kWave) mid[j] = mid_g[j];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t n_tiles = (q + kWave - 1) / kWave;
+    const int64_t wave_stride = (int64_t)gridDim.x * kLdsWaves;
+    for (int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv; t < n_tiles; t += wave_stride) {
+        const TileMasks m = load_tile_edges(Q, q, k, short_max, t, lane, out_v, out_i);
+        if ((m.longs >> lane) & 1) {
+            const Pair best = answer_edges_mid<true, false, TWO>(e, k, mid, mid_shift, mid_nb, m.l, m.r);
+            write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
+        }
+        if (m.shorts) answer_class<G, false>(k, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
+    }
+}
+
+// Coarse rows of a handle (RMQ_BUILD_MID): entry i of row lev is the leftmost
+// minimum of coarse blocks [i, i + 2^lev), that is of blocks [i << shift,
+// min(nb, (i + 2^lev) << shift)), from the block table; entries whose window
+// passes the last coarse block are never read and hold (+inf, INT_MAX).
+__global__ void k_mid_rows(const Pair *__restrict__ st, int nb, int shift, int mid_nb, Pair *__restrict__ mid) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kMidLevels * mid_nb) return;
+    const int lev = t / mid_nb, i = t - lev * mid_nb;
+    if (i + (1 << lev) > mid_nb) {
+        mid[t] = Pair{INFINITY, INT_MAX};
+        return;
+    }
+    const int64_t a = (int64_t)i << shift;
+    const int64_t end = (int64_t)(i + (1 << lev)) << shift;
+    const int64_t b = end < nb ? end : nb;
+    mid[t] = table_min(st, (uint32_t)nb, a, (int)(b - a));
+}
+
 // One tile of a handle with edge tables, its block sparse table in LDS: the
 // answer of this lane's query in registers -- (NaN, -1) for an invalid query,
 // the O(1) edge-table answer for a long range, and for a scanned range the
@@ -563,6 +607,20 @@ void launch_edges_lds(int out, int grid, size_t lds, hipStream_t s, const Blocks
     }
 }
 
+template <bool TWO>
+void launch_edges_mid(int g, int grid, hipStream_t s, const rmq_handle *h, const Blocks &k, const int2 *Q, int64_t q,
+                      int sm, float *val, int32_t *idx) {
+    const dim3 wg(kLdsWaves * kWave);
+    const Pair *m = h->mid;
+    const int sh = h->mid_shift, mn = h->mid_nb;
+    switch (g) {
+    case 4: k_query_edges_mid<4, TWO><<<grid, wg, 0, s>>>(k, h->edges, m, sh, mn, Q, q, sm, val, idx); break;
+    case 8: k_query_edges_mid<8, TWO><<<grid, wg, 0, s>>>(k, h->edges, m, sh, mn, Q, q, sm, val, idx); break;
+    case 16: k_query_edges_mid<16, TWO><<<grid, wg, 0, s>>>(k, h->edges, m, sh, mn, Q, q, sm, val, idx); break;
+    default: k_query_edges_mid<64, TWO><<<grid, wg, 0, s>>>(k, h->edges, m, sh, mn, Q, q, sm, val, idx); break;
+    }
+}
+
 int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_opts_t &o, float *val, int32_t *idx,
                 hipStream_t s) {
     int grid = tiles_grid(q, 8);
@@ -584,6 +642,17 @@ int launch_long(const rmq_handle *h, const int2 *Q, int64_t q, const rmq_query_o
             case 16: launch_edges_lds<16>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
             default: launch_edges_lds<64>(out, g2, lds, s, k, h->edges, p, Q, q, sm, val, idx); break;
             }
+            return hip_rc(hipGetLastError());
+        }
+        if (h->mid) {
+            const int64_t tiles = (q + kWave - 1) / kWave;
+            int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + kLdsWaves - 1) / kLdsWaves,
+                                                                 (int64_t)device_cus() * 2));
+            if (o.long_wgs > 0) g2 = std::min(g2, o.long_wgs);
+            if (two_stage(h))
+                launch_edges_mid<true>(o.group_lanes, g2, s, h, k, Q, q, sm, val, idx);
+            else
+                launch_edges_mid<false>(o.group_lanes, g2, s, h, k, Q, q, sm, val, idx);
             return hip_rc(hipGetLastError());
         }
         if (two_stage(h)) {
@@ -728,8 +797,7 @@ int build_edges(rmq_handle *h, const HandleSizes &z, hipStream_t s) {
 RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out) {
     if (!out) return RMQ_ERR_INVALID_ARG;
     *out = nullptr;
-    if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES) ||
-        (reinterpret_cast<uintptr_t>(d_A) & 15))
+    if (!d_A || n <= 0 || n > INT32_MAX || bs <= 0 || !build_flags_ok(flags) || (reinterpret_cast<uintptr_t>(d_A) & 15))
         return RMQ_ERR_INVALID_ARG;
     const HandleSizes z = handle_sizes(n, bs, flags);
     const int64_t nb = z.nb;
@@ -766,6 +834,17 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
     }
     h->flags = flags;
     if (rc == RMQ_OK && (flags & RMQ_BUILD_EDGES)) rc = build_edges(h, z, s);
+    if (rc == RMQ_OK && (flags & RMQ_BUILD_MID)) {
+        h->mid_shift = z.mid_shift;
+        h->mid_nb = z.mid_nb;
+        h->mid_bytes = z.mid_bytes;
+        rc = hip_rc(hipMalloc(&h->mid, (size_t)z.mid_bytes));
+        if (rc == RMQ_OK) {
+            k_mid_rows<<<(unsigned)((kMidLevels * z.mid_nb + 255) / 256), 256, 0, s>>>(h->st, (int)nb, z.mid_shift,
+                                                                                     z.mid_nb, h->mid);
+            rc = hip_rc(hipGetLastError());
+        }
+    }
     // the sparse table over sub-block minima (32 elements, restarting at
     // every block start; row 0 the minima) for long scans: the edge tables'
     // when built, else an allocation of its own, 8 bytes per 32 elements and
@@ -792,6 +871,7 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
         if (h->st) (void)hipFree(h->st);
         if (h->edge_mem) (void)hipFree(h->edge_mem);
         if (h->sub_mem) (void)hipFree(h->sub_mem);
+        if (h->mid) (void)hipFree(h->mid);
         delete h;
         return rc;
     }
@@ -840,6 +920,7 @@ RMQ_API void rmq_free(rmq_handle_t *h) {
     if (h->st) (void)hipFree(h->st);
     if (h->edge_mem) (void)hipFree(h->edge_mem);
     if (h->sub_mem) (void)hipFree(h->sub_mem);
+    if (h->mid) (void)hipFree(h->mid);
     delete h;
 }
 
@@ -850,12 +931,12 @@ RMQ_API int rmq_handle_info(const rmq_handle_t *h, int64_t *n, int32_t *bs, int6
     if (bs) *bs = h->bs;
     if (nb) *nb = h->nb;
     if (levels) *levels = h->levels;
-    if (bytes) *bytes = h->st_bytes + h->edge_bytes + h->sub_bytes;
+    if (bytes) *bytes = h->st_bytes + h->edge_bytes + h->sub_bytes + h->mid_bytes;
     return RMQ_OK;
 }
 
 RMQ_API int rmq_handle_bytes(int64_t n, int32_t bs, uint32_t flags, int64_t *bytes) {
-    if (!bytes || n <= 0 || n > INT32_MAX || bs <= 0 || (flags & ~RMQ_BUILD_EDGES)) return RMQ_ERR_INVALID_ARG;
+    if (!bytes || n <= 0 || n > INT32_MAX || bs <= 0 || !build_flags_ok(flags)) return RMQ_ERR_INVALID_ARG;
     *bytes = handle_sizes(n, bs, flags).total();
     return RMQ_OK;
 }
@@ -870,7 +951,8 @@ RMQ_API int rmq_query_kernels(const rmq_handle_t *h, const rmq_query_opts_t *opt
     switch (long_pass(h)) {
     case LongPass::kEdgesLds: w = snprintf(buf, (size_t)len, "k_query_edges_lds<%d, 3>", o.group_lanes); break;
     case LongPass::kEdges:
-        w = snprintf(buf, (size_t)len, "k_query_edges<%d, %s>", o.group_lanes, two_stage(h) ? "true" : "false");
+        w = snprintf(buf, (size_t)len, h->mid ? "k_query_edges_mid<%d, %s>" : "k_query_edges<%d, %s>", o.group_lanes,
+                     two_stage(h) ? "true" : "false");
         break;
     default:
         w = snprintf(buf, (size_t)len, "k_query_tiles<%d, false>;k_query_tiles<%d, true>", o.group_lanes,
@@ -1037,7 +1119,7 @@ namespace {
 
 // Cache-key version: bump whenever the tuner's search or timing changes, so
 // picks made by another tuner never match.
-constexpr const char *kTuneVersion = "t2";
+constexpr const char *kTuneVersion = "t3";
 constexpr int kSigWindow = 64;     // elements per sampled window of the array
 constexpr int kSigWindows = 4096;  // windows sampled, evenly spaced
 
@@ -1161,11 +1243,11 @@ int time_pass(const rmq_handle *h, const int32_t *d_Q, int64_t q, float *d_val, 
 
 }  // namespace
 
-RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list,
-                     int32_t n_bs, float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s,
-                     rmq_tune_result_t *out) {
+RMQ_API int rmq_tune_ex(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list,
+                        int32_t n_bs, uint32_t flags_allowed, float *d_val, int32_t *d_idx, const char *cache_path,
+                        hipStream_t s, rmq_tune_result_t *out) {
     if (!out || !d_A || n <= 0 || n > INT32_MAX || q <= 0 || !d_Q || (!d_val && !d_idx) || n_bs < 0 ||
-        (n_bs > 0 && !bs_list))
+        (n_bs > 0 && !bs_list) || !build_flags_ok(flags_allowed))
         return RMQ_ERR_INVALID_ARG;
     *out = rmq_tune_result_t{};
     // cache key: tuner version / n and length percentiles / value signature /
@@ -1188,8 +1270,11 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
             snprintf(cands_tag, sizeof(cands_tag), "/bs#%08x", hsh);
         }
     }
-    snprintf(key, sizeof(key), "%s/%s/%s/out=%s%s", kTuneVersion, lens, vals, d_val && d_idx ? "vi" : (d_val ? "v" : "i"),
-             cands_tag);
+    // so are the build flags it may choose from, when some are excluded
+    char flags_tag[16] = "";
+    if (flags_allowed != (RMQ_BUILD_EDGES | RMQ_BUILD_MID)) snprintf(flags_tag, sizeof(flags_tag), "/f%u", flags_allowed);
+    snprintf(key, sizeof(key), "%s/%s/%s/out=%s%s%s", kTuneVersion, lens, vals,
+             d_val && d_idx ? "vi" : (d_val ? "v" : "i"), cands_tag, flags_tag);
     if (strlen(key) >= sizeof(out->key)) return RMQ_ERR_INVALID_ARG;
     if (cache_lookup(cache_path, key, out)) {
         out->from_cache = 1;
@@ -1227,7 +1312,14 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     std::vector<Cand> stage1;
     for (int32_t bs : cands) {
         if (bs <= 0 || bs > n) continue;
-        for (uint32_t flags : {0u, (uint32_t)RMQ_BUILD_EDGES}) {
+        // coarse rows only where the query pass reads the block table from
+        // global memory (a table that fits in LDS is read from there whole)
+        const HandleSizes zb = handle_sizes(n, bs, 0);
+        const bool mid_useful =
+            !lds_table_fits((size_t)zb.levels * (size_t)zb.nb * sizeof(Pair) +
+                            (sizeof(TileStage) + sizeof(TileResults)) * kLdsWaves);
+        for (uint32_t flags : {0u, (uint32_t)RMQ_BUILD_EDGES, (uint32_t)(RMQ_BUILD_EDGES | RMQ_BUILD_MID)}) {
+            if ((flags & ~flags_allowed) || ((flags & RMQ_BUILD_MID) && !mid_useful)) continue;
             rmq_handle_t *h = nullptr;
             rc = rmq_build(d_A, n, bs, flags, s, &h);
             if (rc == RMQ_ERR_OOM && flags) continue;  // no room for edge tables: plain only
@@ -1348,6 +1440,13 @@ RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q,
     snprintf(out->key, sizeof(out->key), "%s", key);
     cache_store(cache_path, key, *out);
     return RMQ_OK;
+}
+
+RMQ_API int rmq_tune(const float *d_A, int64_t n, const int32_t *d_Q, int64_t q, const int32_t *bs_list,
+                     int32_t n_bs, float *d_val, int32_t *d_idx, const char *cache_path, hipStream_t s,
+                     rmq_tune_result_t *out) {
+    return rmq_tune_ex(d_A, n, d_Q, q, bs_list, n_bs, RMQ_BUILD_EDGES | RMQ_BUILD_MID, d_val, d_idx, cache_path, s,
+                       out);
 }
 
 // ===========================================================================

@@ -609,6 +609,41 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &
     return answer_in_block<NT_PAIRS>(e, k, l, r, bl);
 }
 
+// Coarse blocks of 2^shift blocks, at most kMidMaxBlocks of them, with the
+// first kMidLevels rows of a sparse table over their minima: small enough
+// to sit in LDS beside two workgroups' staging, so that a range's superset
+// test over coarse blocks costs no global read.
+constexpr int kMidLevels = 2;
+constexpr int kMidMaxBlocks = 3072;
+
+// answer_edges with a superset test over the coarse blocks first, from the
+// LDS rows `mid` (kMidLevels rows of mid_nb entries): the range's coarse
+// blocks cl..cr, when they are at most 2^kMidLevels - 1, give a window whose
+// leftmost minimum answers the range when it lies in [l, r]; otherwise, and
+// for a range inside one block, answer_edges as it is.
+template <bool LINES, bool NT_PAIRS, bool TWO_STAGE>
+__device__ __forceinline__ Pair answer_edges_mid(const EdgeTables &e, const Blocks &k, const Pair *mid, int mid_shift,
+                                                 int mid_nb, int l, int r) {
+    const int bl = block_of(l, k.div), br = block_of(r, k.div);
+    if (bl != br) {
+        const int cl = bl >> mid_shift, cr = br >> mid_shift;
+        const int c = cr - cl + 1;
+        if (c < (1 << kMidLevels)) {
+            const int lev = 31 - __clz(c);
+            const Pair *row = mid + lev * mid_nb;
+            const Pair x = row[cl];
+            const Pair y = row[cr - (1 << lev) + 1];
+            const Pair s = (y.v < x.v) ? y : x;
+            if (s.i >= l && s.i <= r) return s;
+        }
+        if constexpr (TWO_STAGE) return answer_two_stage<NT_PAIRS>(e, k, l, r, bl, br);
+        const Pair sup = superset_min<LINES>(k.st, k.nb, bl, br);
+        if (sup.i >= l && sup.i <= r) return sup;
+        return answer_failed<NT_PAIRS>(e, k, l, r, bl, br);
+    }
+    return answer_in_block<NT_PAIRS>(e, k, l, r, bl);
+}
+
 // Per-wave LDS staging of one tile's queries of a class.
 struct TileStage {
     int l[kWave];
@@ -743,10 +778,16 @@ inline int table_levels(int64_t count) {
 //           and two per-sub-block exclusive scans used by the build.
 struct HandleSizes {
     int64_t nb, subs;
-    int32_t levels, nsub, levels_s;
-    int64_t st_bytes, sub_bytes, edge_bytes;
-    int64_t total() const { return st_bytes + sub_bytes + edge_bytes; }
+    int32_t levels, nsub, levels_s, mid_shift, mid_nb;
+    int64_t st_bytes, sub_bytes, edge_bytes, mid_bytes;
+    int64_t total() const { return st_bytes + sub_bytes + edge_bytes + mid_bytes; }
 };
+
+// rmq_build flags: RMQ_BUILD_EDGES, optionally with RMQ_BUILD_MID.
+inline bool build_flags_ok(uint32_t flags) {
+    if (flags & ~(uint32_t)(RMQ_BUILD_EDGES | RMQ_BUILD_MID)) return false;
+    return !(flags & RMQ_BUILD_MID) || (flags & RMQ_BUILD_EDGES);
+}
 
 inline HandleSizes handle_sizes(int64_t n, int32_t bs, uint32_t flags) {
     HandleSizes z{};
@@ -761,6 +802,11 @@ inline HandleSizes handle_sizes(int64_t n, int32_t bs, uint32_t flags) {
         z.edge_bytes = (int64_t)sizeof(Pair) * (per_elem * n + (int64_t)z.levels_s * z.subs + 2 * z.subs);
     } else if (bs > kSub) {
         z.sub_bytes = (int64_t)sizeof(Pair) * z.subs * z.levels_s;
+    }
+    if (flags & RMQ_BUILD_MID) {
+        while (((z.nb - 1) >> z.mid_shift) + 1 > kMidMaxBlocks) ++z.mid_shift;
+        z.mid_nb = (int32_t)(((z.nb - 1) >> z.mid_shift) + 1);
+        z.mid_bytes = (int64_t)sizeof(Pair) * kMidLevels * z.mid_nb;
     }
     return z;
 }
@@ -797,6 +843,10 @@ struct rmq_handle {
     int32_t nsub;            // sub-blocks per block, ceil(bs / 32)
     void *sub_mem;           // own allocation when built without edge tables
     int64_t sub_bytes;
+    rmqx::Pair *mid;         // RMQ_BUILD_MID: kMidLevels rows of mid_nb coarse entries, else null
+    int32_t mid_shift;       // a coarse block is 2^mid_shift blocks
+    int32_t mid_nb;          // coarse blocks, ceil(nb / 2^mid_shift) <= kMidMaxBlocks
+    int64_t mid_bytes;
 };
 
 namespace rmqx {

@@ -284,9 +284,11 @@ def check_queries(Q, n: int, stream=None) -> int:
 
 
 def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool = False, stream=None,
-         want_value: bool = True, want_index: bool = True) -> dict:
-    """Autotuner pick (rmq_tune) for this array and a sample of its queries,
-    timing the outputs the caller will ask for (value and/or index).
+         want_value: bool = True, want_index: bool = True, flags_allowed: int = N.BUILD_EDGES | N.BUILD_MID) -> dict:
+    """Autotuner pick (rmq_tune_ex) for this array and a sample of its queries,
+    timing the outputs the caller will ask for (value and/or index), among
+    the build flags in flags_allowed (N.BUILD_EDGES for handles that go into
+    a Plan or Server, which read no coarse rows).
 
     With a cache file, a pick stored for the same key (tuner version, n,
     length percentiles, value signature, outputs) is returned without timing
@@ -305,11 +307,11 @@ def tune(A, Q_sample, bs_list=None, cache_path: str | None = None, refresh: bool
     arr = (ctypes.c_int32 * len(bs_list))(*bs_list) if bs_list else None
     res = N.TuneResult()
     use_cache = cache_path if (cache_path and not refresh) else None
-    rc = N.lib().rmq_tune(A.data_ptr(), A.numel(), Q_sample.data_ptr(), q, arr, len(bs_list) if bs_list else 0,
-                          v_scr.data_ptr() if v_scr is not None else None,
-                          i_scr.data_ptr() if i_scr is not None else None, use_cache.encode() if use_cache else None,
-                          _stream_ptr(stream), ctypes.byref(res))
-    N.check(rc, "rmq_tune")
+    rc = N.lib().rmq_tune_ex(A.data_ptr(), A.numel(), Q_sample.data_ptr(), q, arr, len(bs_list) if bs_list else 0,
+                             flags_allowed, v_scr.data_ptr() if v_scr is not None else None,
+                             i_scr.data_ptr() if i_scr is not None else None,
+                             use_cache.encode() if use_cache else None, _stream_ptr(stream), ctypes.byref(res))
+    N.check(rc, "rmq_tune_ex")
     o = res.opts
     if cache_path and refresh:  # same line format rmq_tune writes
         with open(cache_path, "a") as f:

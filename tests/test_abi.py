@@ -45,6 +45,31 @@ def test_handle_bytes_without_a_gpu():
     assert R.handle_bytes(1 << 20, 1024, R.BUILD_EDGES) - R.handle_bytes(1 << 20, 1024, 0) >= 24 << 20
 
 
+def test_coarse_rows_flag_and_size_without_a_gpu():
+    """RMQ_BUILD_MID is accepted only with RMQ_BUILD_EDGES, and adds two rows
+    of 8-byte entries over coarse blocks of 2^s blocks, s the smallest that
+    leaves at most 3072 of them: the same rule bench.py's byte model uses to
+    replay the coarse test (bench.mid_shift_of), checked here against the
+    library's own accounting for block counts around each power of two."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    lib = R.lib()
+    b = ctypes.c_int64()
+    assert lib.rmq_handle_bytes(1000, 4, R.BUILD_MID, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(1000, 4, R.BUILD_EDGES | R.BUILD_MID | 4, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
+    assert lib.rmq_handle_bytes(1000, 4, R.BUILD_EDGES | R.BUILD_MID, ctypes.byref(b)) == R.RMQ_OK
+    for nb in (1, 2, 3071, 3072, 3073, 6144, 6145, 97657, 1 << 20, (1 << 20) + 1, 12_500_000, (1 << 30) + 3):
+        bs = 2
+        n = min(nb * bs, 2 ** 31 - 1)
+        nb_real = -(-n // bs)
+        s = bench.mid_shift_of(nb_real)
+        coarse = ((nb_real - 1) >> s) + 1
+        assert coarse <= 3072 and (s == 0 or ((nb_real - 1) >> (s - 1)) + 1 > 3072)
+        extra = R.handle_bytes(n, bs, R.BUILD_EDGES | R.BUILD_MID) - R.handle_bytes(n, bs, R.BUILD_EDGES)
+        assert extra == 2 * 8 * coarse, (nb_real, s, extra)
+
+
 def test_strerror_and_version():
     assert R.strerror(R.RMQ_OK) == "ok"
     for code in (R.RMQ_ERR_INVALID_ARG, R.RMQ_ERR_OOM, R.RMQ_ERR_HIP, R.RMQ_ERR_DEVICE, R.RMQ_ERR_BAD_QUERY):

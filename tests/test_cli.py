@@ -146,6 +146,17 @@ def test_gpu_edges_and_autotune(alg, tmp_path, gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("alg", [5, 105])
+def test_gpu_coarse_rows(alg, gpu):
+    """--edges --mid builds coarse rows (the block table of 2^14 blocks is
+    read from global memory, so the query pass tries them first); answers
+    pass the on-device brute-force check."""
+    r = run(1 << 20, 1 << 16, -2, alg, "--edges", "--mid", "--bs", 64, "--reps", 2, "--seed", 9, "--check")
+    assert r.returncode == 0 and "pass (0 mismatches)" in r.stdout, r.stdout[-1500:]
+    assert "edge tables, coarse rows" in r.stdout
+
+
+@pytest.mark.gpu
 def test_gpu_autotune_keeps_a_given_block_size(tmp_path, gpu):
     """--tune with --nb (the reference's sweep parameter) tunes edge tables and
     thresholds at that block size only; the row and the pick carry it."""

@@ -67,7 +67,7 @@ def test_batched_config_full_size(tmp_path, gpu):
     torch.cuda.synchronize()
     big = int(np.argmax(ns))
     pick = R.tune(A_views[big], Q_views[big], bs_list=[64, 128, 256, 512, 1024, 4096], cache_path=_cache_copy(tmp_path),
-                  want_value=False)
+                  want_value=False, flags_allowed=R.BUILD_EDGES)
     assert pick["from_cache"], f"batched pick {pick['key']} missed the committed cache"
     print(f"batched full size: {m} arrays, {int(ns.sum())} elements, pick bs={pick['bs']} flags={pick['flags']}",
           flush=True)
@@ -110,7 +110,7 @@ def test_served_config_full_size(tmp_path, gpu):
     big = int(np.argmax(ns))
     sample = torch.cat([R.gen_queries_device(1 << 20, int(ns[big]), lr, 77, device=gpu) for lr in MIX])
     pick = R.tune(arrays[big], sample, bs_list=[16, 32, 64, 128, 256, 1024], cache_path=_cache_copy(tmp_path),
-                  want_value=False)
+                  want_value=False, flags_allowed=R.BUILD_EDGES)
     assert pick["from_cache"], f"served pick {pick['key']} missed the committed cache"
     reqs = []  # (array, pinned queries)
     for k in range(96):

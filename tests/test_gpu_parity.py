@@ -210,7 +210,7 @@ def test_autotuner_pick_and_cache(tmp_path, gpu):
     assert r1["opts"].group_lanes in (4, 8, 16, 64) and r1["opts"].long_lanes in (4, 8, 16, 64)
     r2 = R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)
     assert r2["from_cache"] and r2["bs"] == r1["bs"] and r2["key"] == r1["key"]
-    assert r1["key"].startswith("t2/n2^22.0/") and r1["key"].endswith("/pos0.50/tie0/out=vi/bs=256,1024,4096"), \
+    assert r1["key"].startswith("t3/n2^22.0/") and r1["key"].endswith("/pos0.50/tie0/out=vi/bs=256,1024,4096"), \
         r1["key"]
     # another candidate list asks another question: a key of its own, timed afresh
     r3 = R.tune(A_dev, Q_dev, bs_list=[1024], cache_path=cache)
@@ -325,7 +325,8 @@ def test_headline_full_size(lr, flags, gpu):
 
 
 # The kernels the committed MI355X picks run for the three headline batches.
-HEADLINE_KERNELS = {-1: ["k_query_edges_lds<16, 3>"], -2: ["k_query_edges<4, false>"], -3: ["k_query_edges<8, true>"]}
+HEADLINE_KERNELS = {-1: ["k_query_edges_lds<16, 3>"], -2: ["k_query_edges_mid<8, false>"],
+                    -3: ["k_query_edges_mid<16, true>"]}
 
 
 @pytest.mark.parametrize("lr", [-1, -2, -3])
@@ -333,8 +334,8 @@ def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
     """The configurations the headline runs: n=1e8, each length distribution
     at the block size, edge tables and thresholds of the committed MI355X
     tuner cache (the LDS-table kernel for uniform lengths, the global-table
-    kernel with its line-aware superset test for the lognormal ones), every
-    answer checked against the O(1) oracle."""
+    kernel with coarse rows in LDS and its line-aware superset test for the
+    lognormal ones), every answer checked against the O(1) oracle."""
     import shutil
     n, q = 100_000_000, 1 << 26
     A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
@@ -359,9 +360,10 @@ def test_committed_tuner_picks_full_size(lr, tmp_path, gpu):
     assert_exact(v, i, fv, fi, f"lr={lr} pick={pick['key']} bs={pick['bs']} kernels={kern}")
 
 
-# The kernels the committed picks restricted to bs = 2^15 run (edge tables for all three).
-LITERAL_BS_KERNELS = {-1: ["k_query_edges<4, false>"], -2: ["k_query_edges<16, false>"],
-                      -3: ["k_query_edges<16, false>"]}
+# The kernels the committed picks restricted to bs = 2^15 run (edge tables for all three; coarse rows,
+# here one block each, for the uniform and lognormal 2^14 batches).
+LITERAL_BS_KERNELS = {-1: ["k_query_edges_mid<4, false>"], -2: ["k_query_edges_mid<16, false>"],
+                      -3: ["k_query_edges<8, false>"]}
 
 
 @pytest.mark.parametrize("lr", [-1, -2, -3])

@@ -128,14 +128,23 @@ def lazy_two_stage(A, bs, bm, l, r):
     return best, 2, False, reads
 
 
-def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False):
+def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False, mid_shift=None):
     """(answer, parts computed, superset hit, sparse-table entries read) for
     lB < rB (a window's minimum costs two entries, a widened window one),
     following rules
     1-3.  left_counts: a computed left part joins the best so far before the
     right part is decided (the edge-table path reads it as one pair); False
     for the scan path, whose left scan is spread over a lane group.
-    two_stage: the large-table path (lazy_two_stage), always line-aware."""
+    two_stage: the large-table path (lazy_two_stage), always line-aware.
+    mid_shift: rule 1 first over coarse blocks of 2^mid_shift blocks (rows in
+    LDS, no table read) when the range's blocks lie in at most 3 of them."""
+    if mid_shift is not None:
+        lB, rB = l // bs, r // bs
+        cl, cr = lB >> mid_shift, rB >> mid_shift
+        if cr - cl < 3:
+            sup = blocks_min(bm, cl << mid_shift, min(len(bm), (cr + 1) << mid_shift) - 1)
+            if l <= sup[1] <= r:
+                return sup, 0, True, 0
     if two_stage:
         return lazy_two_stage(A, bs, bm, l, r)
     lB, rB = l // bs, r // bs
@@ -179,21 +188,24 @@ def arrays(rng):
     return out
 
 
-@pytest.mark.parametrize("left_counts,lines,two_stage", [(True, False, False), (False, False, False),
-                                                         (True, True, False), (True, True, True)])
+@pytest.mark.parametrize("left_counts,lines,two_stage,mid_shift",
+                         [(True, False, False, None), (False, False, False, None), (True, True, False, None),
+                          (True, True, True, None), (True, True, False, 0), (True, True, False, 1),
+                          (True, True, False, 2), (True, True, True, 1), (True, True, True, 2)])
 @pytest.mark.parametrize("bs", [1, 2, 3, 8, 16, 33])
-def test_rules_give_the_leftmost_minimum(bs, left_counts, lines, two_stage):
+def test_rules_give_the_leftmost_minimum(bs, left_counts, lines, two_stage, mid_shift):
     """lines: the superset window grows to a level-(lev+1) window when the two
     level-lev entries lie in different 128-byte lines (any window of whole
     blocks that covers [l, r] is a valid superset); two_stage: a failed test
-    is retried with the failing side cut off."""
+    is retried with the failing side cut off; mid_shift: a superset test over
+    coarse blocks (the last one partial) comes first."""
     for A in arrays(np.random.default_rng(bs)):
         bm = block_mins(A, bs)
         for l in range(len(A)):
             for r in range(l, len(A)):
                 if l // bs == r // bs:
                     continue
-                got = lazy(A, bs, bm, l, r, left_counts, lines, two_stage)[0]
+                got = lazy(A, bs, bm, l, r, left_counts, lines, two_stage, mid_shift)[0]
                 exp = brute(A, l, r)
                 assert got[1] == exp[1] and np.float32(got[0]).tobytes() == np.float32(exp[0]).tobytes(), \
                     (A.tolist(), bs, l, r, got, exp)
@@ -217,14 +229,16 @@ def test_superset_rule_inside_one_block_over_sub_blocks():
                         assert sup == brute(A, l, r), (bs, l, r)
 
 
-@pytest.mark.parametrize("edges,short_max,lines,two_stage", [(False, 32, False, False), (True, 32, False, False),
-                                                             (True, 1, False, False), (True, 1, True, False),
-                                                             (True, 32, True, True), (True, 1, True, True)])
-def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
+@pytest.mark.parametrize("edges,short_max,lines,two_stage,mid_shift",
+                         [(False, 32, False, False, None), (True, 32, False, False, None),
+                          (True, 1, False, False, None), (True, 1, True, False, None),
+                          (True, 32, True, True, None), (True, 1, True, True, None),
+                          (True, 1, True, False, 1), (True, 32, True, False, 3), (True, 1, True, True, 2)])
+def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage, mid_shift):
     """With edge tables a range inside one 32-element sub-block (one block
     when bs <= 32) is scanned whatever short_max is; every other range longer
     than short_max takes the O(1) path."""
-    rng = np.random.default_rng(11 + edges + short_max + 5 * lines + 7 * two_stage)
+    rng = np.random.default_rng(11 + edges + short_max + 5 * lines + 7 * two_stage + 13 * (mid_shift or 0))
     for trial in range(16):
         n = int(rng.integers(100, 2500))
         bs = int(rng.choice([8, 24, 64, 128, 256] if edges else [4, 16, 37, 64, 256]))
@@ -255,7 +269,8 @@ def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
                     gathers += 0 if hit else 2
                     reads += 2 + (0 if hit or (r - base) // 32 - (l - base) // 32 <= 1 else 2)
                 continue
-            _, k, _, rd = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines, two_stage=two_stage)
+            _, k, _, rd = lazy(A, bs, bm, l, r, left_counts=edges, lines=lines, two_stage=two_stage,
+                               mid_shift=mid_shift)
             parts += 2
             need += k
             reads += rd
@@ -263,7 +278,7 @@ def test_byte_model_replays_the_rules(edges, short_max, lines, two_stage):
         lvl = bench.BlockLevel(torch.from_numpy(A), bs)
         sub = bench.BlockLevel(torch.from_numpy(A), 32) if edges else None
         ab = bench.algorithmic_bytes(torch.from_numpy(Q), lvl, short_max, edges=edges, sub=sub,
-                                     A=torch.from_numpy(A), lines=lines, two_stage=two_stage)
+                                     A=torch.from_numpy(A), lines=lines, two_stage=two_stage, mid_shift=mid_shift)
         assert ab["n_short"] == n_short
         assert abs(ab["edge_parts_computed"] - (need / parts if parts else 0.0)) < 1e-12
         assert ab["gathers"] == gathers
