@@ -170,7 +170,7 @@ def left_part_min(A, sub, bs: int, l):
     return v
 
 
-MID_MAX_BLOCKS = 3072  # coarse blocks of an RMQ_BUILD_MID handle at most (kMidMaxBlocks)
+MID_MAX_BLOCKS = 8192  # coarse blocks of an RMQ_BUILD_MID handle at most (kMidMaxBlocks)
 
 
 def mid_shift_of(nb: int) -> int:
@@ -217,10 +217,12 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
     in different 128-byte lines (BlockLevel.superset), as the edge-table
     kernel with its table in global memory does; with two_stage as well, an
     argmin of that entry past block rB is followed by the two entries.
-    mid_shift (edge tables with coarse rows, RMQ_BUILD_MID): a range across
-    blocks whose blocks lie in at most 3 coarse blocks of 2^mid_shift blocks
-    first takes the minimum of those coarse blocks from LDS; when its argmin
-    lies in [l, r] that is the answer, B = 8 + out, with no table read."""
+    mid_shift (edge tables with a coarse row, RMQ_BUILD_MID): a range across
+    blocks whose blocks lie in coarse blocks (2^mid_shift blocks) cl..cr,
+    cr - cl < 4, first takes from LDS the minimum of coarse blocks cl..cr, or
+    of cl and cl+1 when cr - cl < 2 (the last coarse block alone at the end);
+    when its argmin lies in [l, r] that is the answer, B = 8 + out, with no
+    table read."""
     import torch
     bs = lvl.bs
     long_b = short_b = 0.0
@@ -254,9 +256,10 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         if edges and mid_shift is not None:
             cl, cr = lB >> mid_shift, rB >> mid_shift
-            near = cr - cl < 3  # at most 3 coarse blocks: the two LDS rows
+            near = cr - cl < 4  # at most 4 coarse blocks: one or two entries of the LDS row
             a0 = cl << mid_shift
-            end = torch.clamp((cr + 1) << mid_shift, max=lvl.nb)
+            hi = torch.where(cr - cl < 2, cl + 1, cr)
+            end = torch.clamp((hi + 1) << mid_shift, max=lvl.nb)
             ci = lvl.range_min(a0[near], (end - a0)[near])[1]
             hit = torch.zeros_like(near)
             hit[near] = (ci >= l[near]) & (ci <= r[near])

@@ -50,7 +50,7 @@ typedef struct rmq_handle rmq_handle_t;
  * stream so that host and device produce the same array). */
 /* rmq_build flags */
 #define RMQ_BUILD_EDGES 1u /* precompute the edge-block searches (32 B per element): see rmq_build */
-#define RMQ_BUILD_MID 2u   /* with RMQ_BUILD_EDGES: coarse rows for an LDS superset test (48 KB at most) */
+#define RMQ_BUILD_MID 2u   /* with RMQ_BUILD_EDGES: a coarse row for an LDS superset test (64 KB at most) */
 
 #define RMQ_GEN_UNIFORM 0 /* k * 2^-24, k uniform in [0, 2^24): U[0,1) exactly      */
 #define RMQ_GEN_TIES16 1  /* integers 0..15 as float (heavy ties)                  */
@@ -89,13 +89,14 @@ typedef struct rmq_query_times {
  *      (32 bytes per element): every range not inside one 32-element
  *      sub-block (one block when bs <= 32) is then answered by one lane with
  *      a few O(1) lookups instead of edge scans.  RMQ_BUILD_MID (only with
- *      RMQ_BUILD_EDGES) also keeps two rows of a sparse table over coarse
- *      blocks of 2^j blocks, the fewest that make at most 3072 of them; a
- *      query pass whose block table does not fit in LDS copies those rows
- *      into LDS and answers a range from them when the leftmost minimum of
- *      its (at most 3) coarse blocks lies inside it, before any table read
- *      in global memory.  It pays for ranges a sizeable fraction of a coarse
- *      block long (the autotuner decides).
+ *      RMQ_BUILD_EDGES) also keeps one row over coarse blocks of 2^j blocks,
+ *      the fewest that make at most 8192 of them, whose entry i is the
+ *      leftmost minimum of coarse blocks i and i+1; a query pass whose block
+ *      table does not fit in LDS copies that row into LDS and answers a range
+ *      from one or two of its entries when their window (the range's at
+ *      most 4 coarse blocks) has its leftmost minimum inside the range,
+ *      before any table read in global memory.  It pays for ranges a
+ *      sizeable fraction of a coarse block long (the autotuner decides).
  *      Stream-ordered; the handle is usable on stream s at once, on other
  *      streams after s is synchronised. ---- */
 int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, hipStream_t s, rmq_handle_t **out);

@@ -259,17 +259,17 @@ __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_edges(Blocks k, E
     }
 }
 
-// k_query_edges for a handle with coarse rows (RMQ_BUILD_MID): workgroups of
-// kLdsWaves waves, two per CU, first copy the coarse rows into LDS; every
+// k_query_edges for a handle with a coarse row (RMQ_BUILD_MID): workgroups of
+// kLdsWaves waves, two per CU, first copy the coarse row into LDS; every
 // long range then tries the coarse superset test (answer_edges_mid) before
 // the global table.
 template <int G, bool TWO>
 __global__ __launch_bounds__(kLdsWaves *kWave) RMQ_SGPR_BUDGET void k_query_edges_mid(
     Blocks k, EdgeTables e, const Pair *__restrict__ mid_g, int mid_shift, int mid_nb, const int2 *__restrict__ Q,
     int64_t q, int short_max, float *__restrict__ out_v, int32_t *__restrict__ out_i) {
-    __shared__ Pair mid[kMidLevels * kMidMaxBlocks];
+    __shared__ Pair mid[kMidMaxBlocks];
     __shared__ TileStage stage[kLdsWaves];
-    for (int j = threadIdx.x; j < kMidLevels * mid_nb; j += kLdsWaves * kWave) mid[j] = mid_g[j];
+    for (int j = threadIdx.x; j < mid_nb; j += kLdsWaves * kWave) mid[j] = mid_g[j];
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -278,29 +278,23 @@ __global__ __launch_bounds__(kLdsWaves *kWave) RMQ_SGPR_BUDGET void k_query_edge
     for (int64_t t = (int64_t)blockIdx.x * kLdsWaves + wv; t < n_tiles; t += wave_stride) {
         const TileMasks m = load_tile_edges(Q, q, k, short_max, t, lane, out_v, out_i);
         if ((m.longs >> lane) & 1) {
-            const Pair best = answer_edges_mid<true, false, TWO>(e, k, mid, mid_shift, mid_nb, m.l, m.r);
+            const Pair best = answer_edges_mid<true, false, TWO>(e, k, mid, mid_shift, m.l, m.r);
             write_result(out_v, out_i, t * kWave + lane, best.v, best.i);
         }
         if (m.shorts) answer_class<G, false>(k, m.shorts, m, lane, t * kWave, out_v, out_i, stage[wv]);
     }
 }
 
-// Coarse rows of a handle (RMQ_BUILD_MID): entry i of row lev is the leftmost
-// minimum of coarse blocks [i, i + 2^lev), that is of blocks [i << shift,
-// min(nb, (i + 2^lev) << shift)), from the block table; entries whose window
-// passes the last coarse block are never read and hold (+inf, INT_MAX).
+// The coarse row of a handle (RMQ_BUILD_MID): entry i is the leftmost minimum
+// of coarse blocks i and i+1 (the last entry, of the last coarse block), that
+// is of blocks [i << shift, min(nb, (i + 2) << shift)), from the block table.
 __global__ void k_mid_rows(const Pair *__restrict__ st, int nb, int shift, int mid_nb, Pair *__restrict__ mid) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= kMidLevels * mid_nb) return;
-    const int lev = t / mid_nb, i = t - lev * mid_nb;
-    if (i + (1 << lev) > mid_nb) {
-        mid[t] = Pair{INFINITY, INT_MAX};
-        return;
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mid_nb) return;
     const int64_t a = (int64_t)i << shift;
-    const int64_t end = (int64_t)(i + (1 << lev)) << shift;
+    const int64_t end = (int64_t)(i + 2) << shift;
     const int64_t b = end < nb ? end : nb;
-    mid[t] = table_min(st, (uint32_t)nb, a, (int)(b - a));
+    mid[i] = table_min(st, (uint32_t)nb, a, (int)(b - a));
 }
 
 // One tile of a handle with edge tables, its block sparse table in LDS: the
@@ -840,8 +834,8 @@ RMQ_API int rmq_build(const float *d_A, int64_t n, int32_t bs, uint32_t flags, h
         h->mid_bytes = z.mid_bytes;
         rc = hip_rc(hipMalloc(&h->mid, (size_t)z.mid_bytes));
         if (rc == RMQ_OK) {
-            k_mid_rows<<<(unsigned)((kMidLevels * z.mid_nb + 255) / 256), 256, 0, s>>>(h->st, (int)nb, z.mid_shift,
-                                                                                     z.mid_nb, h->mid);
+            k_mid_rows<<<(unsigned)((z.mid_nb + 255) / 256), 256, 0, s>>>(h->st, (int)nb, z.mid_shift, z.mid_nb,
+                                                                        h->mid);
             rc = hip_rc(hipGetLastError());
         }
     }

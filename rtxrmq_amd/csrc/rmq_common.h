@@ -609,30 +609,28 @@ __device__ __forceinline__ Pair answer_edges(const EdgeTables &e, const Blocks &
     return answer_in_block<NT_PAIRS>(e, k, l, r, bl);
 }
 
-// Coarse blocks of 2^shift blocks, at most kMidMaxBlocks of them, with the
-// first kMidLevels rows of a sparse table over their minima: small enough
-// to sit in LDS beside two workgroups' staging, so that a range's superset
+// Coarse blocks of 2^shift blocks, at most kMidMaxBlocks of them, and one
+// row over them: entry i is the leftmost minimum of coarse blocks i and i+1
+// (the last entry, of the last block alone).  64 KB at most: it sits in LDS
+// beside the staging of two 16-wave workgroups per CU, so a range's superset
 // test over coarse blocks costs no global read.
-constexpr int kMidLevels = 2;
-constexpr int kMidMaxBlocks = 3072;
+constexpr int kMidMaxBlocks = 8192;
 
 // answer_edges with a superset test over the coarse blocks first, from the
-// LDS rows `mid` (kMidLevels rows of mid_nb entries): the range's coarse
-// blocks cl..cr, when they are at most 2^kMidLevels - 1, give a window whose
-// leftmost minimum answers the range when it lies in [l, r]; otherwise, and
-// for a range inside one block, answer_edges as it is.
+// LDS row `mid` (mid_nb entries): a range whose blocks lie in coarse blocks
+// cl..cr with cr - cl < 4 takes the entry at cl (one or two coarse blocks)
+// or the entries at cl and cr-1 (three or four); when the window's leftmost
+// minimum lies in [l, r] it is the answer.  Otherwise, and for a range
+// inside one block, answer_edges as it is.
 template <bool LINES, bool NT_PAIRS, bool TWO_STAGE>
 __device__ __forceinline__ Pair answer_edges_mid(const EdgeTables &e, const Blocks &k, const Pair *mid, int mid_shift,
-                                                 int mid_nb, int l, int r) {
+                                                 int l, int r) {
     const int bl = block_of(l, k.div), br = block_of(r, k.div);
     if (bl != br) {
         const int cl = bl >> mid_shift, cr = br >> mid_shift;
-        const int c = cr - cl + 1;
-        if (c < (1 << kMidLevels)) {
-            const int lev = 31 - __clz(c);
-            const Pair *row = mid + lev * mid_nb;
-            const Pair x = row[cl];
-            const Pair y = row[cr - (1 << lev) + 1];
+        if (cr - cl < 4) {
+            const Pair x = mid[cl];
+            const Pair y = mid[cr - cl < 2 ? cl : cr - 1];
             const Pair s = (y.v < x.v) ? y : x;
             if (s.i >= l && s.i <= r) return s;
         }
@@ -806,7 +804,7 @@ inline HandleSizes handle_sizes(int64_t n, int32_t bs, uint32_t flags) {
     if (flags & RMQ_BUILD_MID) {
         while (((z.nb - 1) >> z.mid_shift) + 1 > kMidMaxBlocks) ++z.mid_shift;
         z.mid_nb = (int32_t)(((z.nb - 1) >> z.mid_shift) + 1);
-        z.mid_bytes = (int64_t)sizeof(Pair) * kMidLevels * z.mid_nb;
+        z.mid_bytes = (int64_t)sizeof(Pair) * z.mid_nb;
     }
     return z;
 }
@@ -843,7 +841,7 @@ struct rmq_handle {
     int32_t nsub;            // sub-blocks per block, ceil(bs / 32)
     void *sub_mem;           // own allocation when built without edge tables
     int64_t sub_bytes;
-    rmqx::Pair *mid;         // RMQ_BUILD_MID: kMidLevels rows of mid_nb coarse entries, else null
+    rmqx::Pair *mid;         // RMQ_BUILD_MID: mid_nb coarse entries (blocks i and i+1 each), else null
     int32_t mid_shift;       // a coarse block is 2^mid_shift blocks
     int32_t mid_nb;          // coarse blocks, ceil(nb / 2^mid_shift) <= kMidMaxBlocks
     int64_t mid_bytes;

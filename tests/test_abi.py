@@ -46,9 +46,9 @@ def test_handle_bytes_without_a_gpu():
 
 
 def test_coarse_rows_flag_and_size_without_a_gpu():
-    """RMQ_BUILD_MID is accepted only with RMQ_BUILD_EDGES, and adds two rows
+    """RMQ_BUILD_MID is accepted only with RMQ_BUILD_EDGES, and adds one row
     of 8-byte entries over coarse blocks of 2^s blocks, s the smallest that
-    leaves at most 3072 of them: the same rule bench.py's byte model uses to
+    leaves at most 8192 of them: the same rule bench.py's byte model uses to
     replay the coarse test (bench.mid_shift_of), checked here against the
     library's own accounting for block counts around each power of two."""
     import sys
@@ -59,15 +59,15 @@ def test_coarse_rows_flag_and_size_without_a_gpu():
     assert lib.rmq_handle_bytes(1000, 4, R.BUILD_MID, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_handle_bytes(1000, 4, R.BUILD_EDGES | R.BUILD_MID | 4, ctypes.byref(b)) == R.RMQ_ERR_INVALID_ARG
     assert lib.rmq_handle_bytes(1000, 4, R.BUILD_EDGES | R.BUILD_MID, ctypes.byref(b)) == R.RMQ_OK
-    for nb in (1, 2, 3071, 3072, 3073, 6144, 6145, 97657, 1 << 20, (1 << 20) + 1, 12_500_000, (1 << 30) + 3):
+    for nb in (1, 2, 8191, 8192, 8193, 16384, 16385, 97657, 1 << 20, (1 << 20) + 1, 12_500_000, (1 << 30) + 3):
         bs = 2
         n = min(nb * bs, 2 ** 31 - 1)
         nb_real = -(-n // bs)
         s = bench.mid_shift_of(nb_real)
         coarse = ((nb_real - 1) >> s) + 1
-        assert coarse <= 3072 and (s == 0 or ((nb_real - 1) >> (s - 1)) + 1 > 3072)
+        assert coarse <= 8192 and (s == 0 or ((nb_real - 1) >> (s - 1)) + 1 > 8192)
         extra = R.handle_bytes(n, bs, R.BUILD_EDGES | R.BUILD_MID) - R.handle_bytes(n, bs, R.BUILD_EDGES)
-        assert extra == 2 * 8 * coarse, (nb_real, s, extra)
+        assert extra == 8 * coarse, (nb_real, s, extra)
 
 
 def test_strerror_and_version():

@@ -1,6 +1,7 @@
-"""Coarse rows (RMQ_BUILD_MID): a superset test over coarse blocks of 2^s
-blocks, from two sparse-table rows the query kernel keeps in LDS, before the
-block table in global memory (k_query_edges_mid).  Bar: bit-exact against
+"""Coarse row (RMQ_BUILD_MID): a superset test over coarse blocks of 2^s
+blocks, from a row the query kernel keeps in LDS (entry i: the leftmost
+minimum of coarse blocks i and i+1), before the block table in global memory
+(k_query_edges_mid).  Bar: bit-exact against
 CPU BASE brute force (leftmost minimum, value bits), on arrays whose ties,
 +-0 and +inf runs make the leftmost rule matter, with ranges that equal,
 straddle and end inside coarse blocks, including the partial last one.
@@ -44,7 +45,7 @@ def tie_array(n, seed):
 
 def coarse_queries(n, bs, q, seed, max_coarse=6):
     """Lengths log-uniform up to max_coarse coarse blocks; ranges that are
-    exactly 1..3 coarse blocks; ranges that straddle a coarse boundary by one
+    exactly 1..4 coarse blocks; ranges that straddle a coarse boundary by one
     element on each side; ranges ending at the array's end (the partial last
     coarse block) and starting at 0."""
     nb = -(-n // bs)
@@ -55,7 +56,7 @@ def coarse_queries(n, bs, q, seed, max_coarse=6):
     r = np.minimum(l + ln - 1, n - 1)
     parts = [np.stack([l, r], 1)]
     starts = np.arange(0, n, coarse)
-    for c in (1, 2, 3):
+    for c in (1, 2, 3, 4):
         ll = starts[rng.integers(0, starts.size, 2000)]
         parts.append(np.stack([ll, np.minimum(ll + c * coarse - 1, n - 1)], 1))
         parts.append(np.stack([np.maximum(ll - 1, 0), np.minimum(ll + c * coarse, n - 1)], 1))
@@ -87,8 +88,9 @@ def test_kernel_names(gpu):
 @pytest.mark.parametrize("kind", ["ties", "uniform"])
 def test_matches_brute_force(kind, bs, group_lanes, gpu):
     """n = 2,000,003 (a partial last block and coarse block): bs=16 and 64
-    give coarse blocks of 1024 elements (shift 6, 4), bs=100 of 800 (shift
-    3), bs=1024 of one block (shift 0: the coarse test is the block test)."""
+    give coarse blocks of 256 elements (shift 4, 2), bs=100 of 400 (shift 2),
+    bs=1024 of one block (shift 0: the coarse test covers one or two blocks
+    with one entry)."""
     n = 2_000_003
     A = tie_array(n, bs) if kind == "ties" else O.gen_array(n, 7 + bs, "uniform")
     Q, coarse = coarse_queries(n, bs, 1 << 18, bs + group_lanes)
@@ -111,8 +113,8 @@ def test_matches_brute_force(kind, bs, group_lanes, gpu):
 @pytest.mark.parametrize("bs", [2, 8])
 def test_large_table_path(bs, gpu):
     """Block tables over 64 MB take the phased path after a failed coarse
-    test (k_query_edges_mid<G, true>): n = 2^24 + 5, coarse blocks of 2^12
-    (bs=2) or 2^9 (bs=8) blocks, 8192 or 4096 elements."""
+    test (k_query_edges_mid<G, true>): n = 2^24 + 5, coarse blocks of 2^11
+    (bs=2) or 2^9 (bs=8) blocks, 4096 elements either way."""
     n = (1 << 24) + 5
     A = tie_array(n, 50 + bs)
     Q, coarse = coarse_queries(n, bs, 1 << 19, 60 + bs, max_coarse=4)
@@ -126,9 +128,9 @@ def test_large_table_path(bs, gpu):
 
 
 def test_headline_size_lognormal_medium(gpu):
-    """n=1e8, q=2^26 lognormal 2^14 lengths at bs=1024 (coarse blocks of 32
+    """n=1e8, q=2^26 lognormal 2^14 lengths at bs=1024 (coarse blocks of 16
     blocks): every answer against the fast oracle, and identical to the same
-    handle built without coarse rows."""
+    handle built without the coarse row."""
     n, q, bs = 100_000_000, 1 << 26, 1024
     A_dev = R.gen_array(n, 1321, "uniform", device=gpu)
     Q = R.gen_queries(q, n, -2, 1319)

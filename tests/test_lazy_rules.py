@@ -136,13 +136,16 @@ def lazy(A, bs, bm, l, r, left_counts=True, lines=False, two_stage=False, mid_sh
     right part is decided (the edge-table path reads it as one pair); False
     for the scan path, whose left scan is spread over a lane group.
     two_stage: the large-table path (lazy_two_stage), always line-aware.
-    mid_shift: rule 1 first over coarse blocks of 2^mid_shift blocks (rows in
-    LDS, no table read) when the range's blocks lie in at most 3 of them."""
+    mid_shift: rule 1 first over coarse blocks of 2^mid_shift blocks (a row in
+    LDS whose entry i covers coarse blocks i and i+1; no table read) when the
+    range's blocks lie in at most 4 of them: coarse blocks cl..cr, or cl and
+    cl+1 when cr - cl < 2."""
     if mid_shift is not None:
         lB, rB = l // bs, r // bs
         cl, cr = lB >> mid_shift, rB >> mid_shift
-        if cr - cl < 3:
-            sup = blocks_min(bm, cl << mid_shift, min(len(bm), (cr + 1) << mid_shift) - 1)
+        if cr - cl < 4:
+            hi = cl + 1 if cr - cl < 2 else cr
+            sup = blocks_min(bm, cl << mid_shift, min(len(bm), (hi + 1) << mid_shift) - 1)
             if l <= sup[1] <= r:
                 return sup, 0, True, 0
     if two_stage:
