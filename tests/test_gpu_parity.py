@@ -223,6 +223,13 @@ def test_autotuner_pick_and_cache(tmp_path, gpu):
     r4 = R.tune(A_dev, Q_dev, bs_list=long_list, cache_path=cache)
     assert not r4["from_cache"] and r4["key"].endswith(f"/out=vi/bs#{h:08x}"), r4["key"]
     assert R.tune(A_dev, Q_dev, bs_list=[256, 1024, 4096], cache_path=cache)["bs"] == r1["bs"]
+    # a restricted set of build flags is a question of its own too (rmq_tune_ex)
+    r5 = R.tune(A_dev, Q_dev, bs_list=[16, 64], cache_path=cache, flags_allowed=R.BUILD_EDGES)
+    assert not r5["from_cache"] and r5["key"].endswith("/bs=16,64/f1") and not r5["flags"] & R.BUILD_MID, r5
+    r6 = R.tune(A_dev, Q_dev, bs_list=[16, 64], cache_path=cache, flags_allowed=0)
+    assert r6["key"].endswith("/bs=16,64/f0") and r6["flags"] == 0, r6
+    with pytest.raises(R.RMQError):
+        R.tune(A_dev, Q_dev, bs_list=[16, 64], flags_allowed=R.BUILD_MID)  # coarse rows need edge tables
     with R.RMQIndex(A_dev, r1["bs"]) as ix:
         v, i = ix.query(Q_dev, opts=r1["opts"])
         torch.cuda.synchronize()
