@@ -663,11 +663,12 @@ struct TileResults {
 // wave per query, queries taken from the mask in lane order.  G < 64: the
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
-template <int G, bool LONG, bool NT = true, bool TO_LDS = false>
+// U_SHORT: float4 loads per lane in flight for short ranges with G < 64.
+template <int G, bool LONG, bool NT = true, bool TO_LDS = false, int U_SHORT = 1>
 __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage, TileResults *res = nullptr) {
-    constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : 1);
+    constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : U_SHORT);
     if constexpr (LONG) {
         // One lane per query first: a range across blocks whose blocks'
         // minimum has its argmin inside the range is answered here by two
