@@ -422,10 +422,11 @@ __global__ __launch_bounds__(kWG) RMQ_SGPR_BUDGET void k_query_tiles(Blocks k, c
     const int64_t wave_stride = (int64_t)gridDim.x * kWavesPerWG;
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerWG + wv; t < n_tiles; t += wave_stride) {
         const TileMasks m = load_tile(Q, q, k.n, short_max, t, lane, !LONG, out_v, out_i);
-        // short ranges: two float4 loads per lane in flight (a ~100-element
-        // range is then one round trip for a 16-lane group, not two)
-        answer_class<G, LONG, true, false, 2>(k, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
-                                              stage[wv]);
+        // short ranges: up to six float4 loads per lane in flight, each under
+        // its own bounds test, so a range of up to 6*4*G elements is one
+        // round trip for its lane group
+        answer_class<G, LONG, true, false, 6, true>(k, LONG ? m.longs : m.shorts, m, lane, t * kWave, out_v, out_i,
+                                                    stage[wv]);
     }
 }
 

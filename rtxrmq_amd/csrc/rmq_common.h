@@ -127,7 +127,10 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
 // 3 after the last) are scalar loads; the aligned middle is float4 loads, U
 // per lane in flight.  Each lane visits head, middle, tail in index order.
 // Every load lies inside [a, b], so nothing past the array end is touched.
-template <int STRIDE, int U>
+// PRED: the middle's loads go out U at a time, each under its own bounds
+// test, so that a lane with between 1 and U float4s has them all in flight
+// at once (without it, U loads go out only while U remain).
+template <int STRIDE, int U, bool PRED = false>
 __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
     const int ah = (int)min(((int64_t)a + 3) & ~(int64_t)3, (int64_t)INT_MAX);  // a + 3 can pass INT_MAX
     const int bt = (b + 1) & ~3;  // b + 1 <= INT_MAX: b < n <= INT_MAX
@@ -138,14 +141,26 @@ __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, i
     int v = (h_end >> 2) + lane;
     const int v_end = t_beg >> 2;  // exclusive; empty when h_end == t_beg
     if (h_end < t_beg) {
-        for (; v + (U - 1) * STRIDE < v_end; v += U * STRIDE) {
-            float4 x[U];
+        if constexpr (PRED) {
+            for (; v < v_end; v += U * STRIDE) {
+                float4 x[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) x[u] = A4[v + u * STRIDE];
+                for (int u = 0; u < U; ++u)
+                    if (v + u * STRIDE < v_end) x[u] = A4[v + u * STRIDE];
 #pragma unroll
-            for (int u = 0; u < U; ++u) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
+                for (int u = 0; u < U; ++u)
+                    if (v + u * STRIDE < v_end) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
+            }
+        } else {
+            for (; v + (U - 1) * STRIDE < v_end; v += U * STRIDE) {
+                float4 x[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) x[u] = A4[v + u * STRIDE];
+#pragma unroll
+                for (int u = 0; u < U; ++u) upd4(x[u], (v + u * STRIDE) << 2, bv, bi);
+            }
+            for (; v < v_end; v += STRIDE) upd4(A4[v], v << 2, bv, bi);
         }
-        for (; v < v_end; v += STRIDE) upd4(A4[v], v << 2, bv, bi);
     }
     if (lane < b + 1 - t_beg) upd(A[t_beg + lane], t_beg + lane, bv, bi);
 }
@@ -350,11 +365,11 @@ __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi
 // are taken from their minima or scanned only when they can still win.  The
 // group's partial results are combined with (mv, mi) with the index as
 // tie-break, so the leftmost minimum wins.
-template <int G, int U, bool LONG>
+template <int G, int U, bool LONG, bool PRED = false>
 __device__ __forceinline__ void answer_partial(const Blocks &k, int l, int r, int gl, float &bv, int &bi, float &mv,
                                                int &mi) {
     if constexpr (!LONG) {
-        scan_range<G, U>(k.A, l, r, gl, bv, bi);
+        scan_range<G, U, PRED>(k.A, l, r, gl, bv, bi);
     } else {
         const int lB = block_of(l, k.div), rB = block_of(r, k.div);
         if (lB == rB) {
@@ -663,12 +678,15 @@ struct TileResults {
 // wave per query, queries taken from the mask in lane order.  G < 64: the
 // queries are staged in LDS and answered 64/G per round, every lane taking
 // part in each round's DPP reduction.  Must be called by the whole wave.
-// U_SHORT: float4 loads per lane in flight for short ranges with G < 64.
-template <int G, bool LONG, bool NT = true, bool TO_LDS = false, int U_SHORT = 1>
+// U_SHORT: float4 loads per lane in flight for short ranges with G < 64;
+// PRED_SHORT: those loads issued U_SHORT at a time under their own bounds
+// tests (scan_range's PRED).
+template <int G, bool LONG, bool NT = true, bool TO_LDS = false, int U_SHORT = 1, bool PRED_SHORT = false>
 __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const TileMasks &m, int lane, int64_t qbase,
                                              float *__restrict__ out_v, int32_t *__restrict__ out_i,
                                              TileStage &stage, TileResults *res = nullptr) {
     constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : U_SHORT);
+    constexpr bool PRED = !LONG && G < kWave && PRED_SHORT;
     if constexpr (LONG) {
         // One lane per query first: a range across blocks whose blocks'
         // minimum has its argmin inside the range is answered here by two
@@ -727,7 +745,7 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
             int ql = 0;
             if (has) {
                 ql = stage.l[s];
-                answer_partial<G, U, LONG>(k, ql, stage.r[s], gl, bv, bi, mv, mi);
+                answer_partial<G, U, LONG, PRED>(k, ql, stage.r[s], gl, bv, bi, mv, mi);
             }
             group_reduce<G>(bv, bi);
             take_if_better(mv, mi, bv, bi);
