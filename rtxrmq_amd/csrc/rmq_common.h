@@ -342,15 +342,22 @@ __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi
         const int jl = (lo - base) >> kSubShift, jr = (hi - base) >> kSubShift;
         const int whole = jr - jl - 1;
         if (whole >= kHierMin) {
+            // The partial sub-blocks' loads go out U at a time under their
+            // own bounds tests (scan_range's PRED); the whole sub-blocks'
+            // two table entries are loaded before them, so that their round
+            // trip overlaps the scans' (the minimum is a lexicographic
+            // combine, so the order of the combines does not matter).
             const int64_t row = (int64_t)b * k.nsub;
-            scan_range<G, U>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
             if (whole > kHierTable) {
                 const Pair m = table_min(k.sub, (uint32_t)(k.nb * k.nsub), row + jl + 1, whole);
+                scan_range<G, U, true>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
+                scan_range<G, U, true>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
                 take_if_better(m.v, m.i, bv, bi);
             } else {
+                scan_range<G, U, true>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
                 scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
+                scan_range<G, U, true>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
             }
-            scan_range<G, U>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
             return;
         }
     }
