@@ -335,6 +335,45 @@ __device__ __forceinline__ void scan_pairs(const Pair *__restrict__ P, int64_t a
 // their (min, argmin) pairs, 8 bytes per 32 elements, or, beyond kHierTable
 // of them, as two entries of the sparse table over sub-block minima (its
 // rows follow row 0 at a stride of nb * nsub).  Otherwise one direct scan.
+// The split scan_range makes of [a, b]: scalar head [a, h_end), float4s
+// [h_end/4, t_beg/4), scalar tail [t_beg, b].
+struct ScanSplit {
+    int a, b, h_end, t_beg;
+};
+
+__device__ __forceinline__ ScanSplit split_scan(int a, int b) {
+    const int ah = (int)min(((int64_t)a + 3) & ~(int64_t)3, (int64_t)INT_MAX);
+    const int h_end = min(ah, b + 1);
+    return ScanSplit{a, b, h_end, max((b + 1) & ~3, h_end)};
+}
+
+// A scan of at most 32 elements by G lanes with every load issued before the
+// first use (at most 8 float4s: max(1, 8/G) per lane), then combined in index
+// order, as scan_range visits them.
+template <int G>
+struct ShortScan {
+    static constexpr int UM = G >= 8 ? 1 : 8 / G;
+    float h, t;
+    float4 m[UM];
+    __device__ __forceinline__ void issue(const float *__restrict__ A, const ScanSplit &p, int lane) {
+        if (lane < p.h_end - p.a) h = A[p.a + lane];
+        const float4 *__restrict__ A4 = reinterpret_cast<const float4 *>(A);
+        const int v0 = (p.h_end >> 2) + lane, v_end = p.t_beg >> 2;
+#pragma unroll
+        for (int u = 0; u < UM; ++u)
+            if (v0 + u * G < v_end) m[u] = A4[v0 + u * G];
+        if (lane < p.b + 1 - p.t_beg) t = A[p.t_beg + lane];
+    }
+    __device__ __forceinline__ void combine(const ScanSplit &p, int lane, float &bv, int &bi) const {
+        if (lane < p.h_end - p.a) upd(h, p.a + lane, bv, bi);
+        const int v0 = (p.h_end >> 2) + lane, v_end = p.t_beg >> 2;
+#pragma unroll
+        for (int u = 0; u < UM; ++u)
+            if (v0 + u * G < v_end) upd4(m[u], (v0 + u * G) << 2, bv, bi);
+        if (lane < p.b + 1 - p.t_beg) upd(t, p.t_beg + lane, bv, bi);
+    }
+};
+
 template <int G, int U>
 __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi, int gl, float &bv, int &bi) {
     if (k.sub) {
@@ -342,21 +381,28 @@ __device__ __forceinline__ void scan_hier(const Blocks &k, int b, int lo, int hi
         const int jl = (lo - base) >> kSubShift, jr = (hi - base) >> kSubShift;
         const int whole = jr - jl - 1;
         if (whole >= kHierMin) {
-            // The partial sub-blocks' loads go out U at a time under their
-            // own bounds tests (scan_range's PRED); the whole sub-blocks'
-            // two table entries are loaded before them, so that their round
-            // trip overlaps the scans' (the minimum is a lexicographic
-            // combine, so the order of the combines does not matter).
+            // Every load goes out before the first combine: the whole
+            // sub-blocks' two table entries, then both partial sub-blocks'
+            // elements (ShortScan), so that the three parts share one round
+            // trip.  The combines run in index order (left partial, middle,
+            // right partial), as the strict '<' of upd needs.
             const int64_t row = (int64_t)b * k.nsub;
+            const ScanSplit p1 = split_scan(lo, base + (jl << kSubShift) + kSub - 1);
+            const ScanSplit p2 = split_scan(base + (jr << kSubShift), hi);
+            ShortScan<G> s1, s2;
             if (whole > kHierTable) {
                 const Pair m = table_min(k.sub, (uint32_t)(k.nb * k.nsub), row + jl + 1, whole);
-                scan_range<G, U, true>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
-                scan_range<G, U, true>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
+                s1.issue(k.A, p1, gl);
+                s2.issue(k.A, p2, gl);
+                s1.combine(p1, gl, bv, bi);
                 take_if_better(m.v, m.i, bv, bi);
+                s2.combine(p2, gl, bv, bi);
             } else {
-                scan_range<G, U, true>(k.A, lo, base + (jl << kSubShift) + kSub - 1, gl, bv, bi);
+                s1.issue(k.A, p1, gl);
+                s2.issue(k.A, p2, gl);
+                s1.combine(p1, gl, bv, bi);
                 scan_pairs<G, U>(k.sub, row + jl + 1, row + jr - 1, gl, bv, bi);
-                scan_range<G, U, true>(k.A, base + (jr << kSubShift), hi, gl, bv, bi);
+                s2.combine(p2, gl, bv, bi);
             }
             return;
         }
