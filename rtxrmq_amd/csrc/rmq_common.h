@@ -132,6 +132,7 @@ __device__ __forceinline__ uint32_t mask_rank(uint64_t m) {
 // at once (without it, U loads go out only while U remain).
 template <int STRIDE, int U, bool PRED = false>
 __device__ __forceinline__ void scan_range(const float *__restrict__ A, int a, int b, int lane, float &bv, int &bi) {
+    static_assert(STRIDE >= 4, "the scalar head and tail (up to 3 elements each) take one lane per element");
     const int ah = (int)min(((int64_t)a + 3) & ~(int64_t)3, (int64_t)INT_MAX);  // a + 3 can pass INT_MAX
     const int bt = (b + 1) & ~3;  // b + 1 <= INT_MAX: b < n <= INT_MAX
     const int h_end = min(ah, b + 1);
@@ -352,6 +353,7 @@ __device__ __forceinline__ ScanSplit split_scan(int a, int b) {
 // order, as scan_range visits them.
 template <int G>
 struct ShortScan {
+    static_assert(G >= 4, "the scalar head and tail (up to 3 elements each) take one lane per element");
     static constexpr int UM = G >= 8 ? 1 : 8 / G;
     float h, t;
     float4 m[UM];
