@@ -180,6 +180,42 @@ def test_short_ranges_at_every_alignment_and_the_array_end(group_lanes, short_ma
             assert_exact(v, i, ev, ei, f"n={n} bs={bs} flags={flags} G={group_lanes} short_max={short_max}")
 
 
+@pytest.mark.parametrize("long_lanes", [4, 8, 16, 64])
+def test_long_in_block_scans_at_every_sub_block_alignment(long_lanes, gpu):
+    """The long-range path without edge tables scans a range inside one block
+    as a partial 32-element sub-block, the whole sub-blocks between (their
+    minima or two sparse-table entries) and a partial sub-block, with every
+    load issued before the first combine.  Block sizes that are not
+    multiples of 4 (1001, 4099) start their sub-blocks at every offset of a
+    16-byte chunk, so the partials' scalar heads and tails take every length;
+    ranges start and end at every position of their sub-blocks, span 6 to
+    128 sub-blocks, and some end at the array's end (bs=1001); ties and +inf runs make the
+    leftmost rule matter.  Against brute force."""
+    rng = np.random.default_rng(long_lanes)
+    opts = R.query_opts(short_max=1, group_lanes=16, long_lanes=long_lanes)
+    for bs, n in ((1001, 40_037), (4099, 81_983), (333, 20_011)):
+        A = rng.integers(0, 6, n).astype(np.float32)
+        A[(np.arange(n) // 97) % 5 == 1] = np.inf
+        qs = []
+        nb = -(-n // bs)
+        for b in sorted(set(range(0, nb, 3)) | {nb - 1}):  # the partial last block too
+            base = b * bs
+            end = min(base + bs, n) - 1
+            for _ in range(40):
+                l = base + int(rng.integers(0, 64))
+                r = end - int(rng.integers(0, 64))
+                if r - l >= 5 * 32:
+                    qs.append((l, r))
+            for off in range(0, 36):  # every head offset against a fixed far end, and the reverse
+                if base + off < end - 200:
+                    qs.append((base + off, end - 37))
+                    qs.append((base + 37, end - off))
+        Q = np.array(qs, np.int32)
+        ev, ei = O.rmq_brute(A, Q)
+        v, i = run_engine(A, Q, bs, gpu, opts=opts)
+        assert_exact(v, i, ev, ei, f"bs={bs} GL={long_lanes}")
+
+
 def test_query_kernel_names(gpu):
     """rmq_query_kernels names what rmq_query launches: with edge tables the
     LDS-table kernel when the block sparse table fits in LDS (twice per CU),
