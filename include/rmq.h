@@ -61,8 +61,9 @@ typedef struct rmq_handle rmq_handle_t;
 typedef struct rmq_query_opts {
     int32_t short_max;   /* ranges of at most this many elements go to the group scan;
                             with edge tables so does every range inside one 32-element
-                            sub-block (one block when bs <= 32), all others are O(1)  */
-    int32_t group_lanes; /* lanes per short query: 4, 8, 16 or 64 (default 16)          */
+                            sub-block (one block when bs <= 32), all others are O(1)
+                            (default 128)                                              */
+    int32_t group_lanes; /* lanes per short query: 4, 8, 16 or 64 (default 8)           */
     int32_t long_wgs;    /* persistent workgroups for the long-range kernel (0 = auto)  */
     int32_t long_lanes;  /* lanes per long query: 4, 8, 16 or 64 (default 4)            */
 } rmq_query_opts_t;

@@ -1288,8 +1288,11 @@ RMQ_API int rmq_tune_ex(const float *d_A, int64_t n, const int32_t *d_Q, int64_t
     // an array smaller than every candidate is one block
     if (std::none_of(cands.begin(), cands.end(), [&](int32_t b) { return b > 0 && b <= n; }))
         cands.push_back((int32_t)n);
+    // stage 1's lanes and wide short-range threshold, fixed apart from the
+    // query defaults so that one tuner version makes one set of picks
+    constexpr int32_t kBaseShortMax = 1024;
     auto base_opts = [](int32_t) {
-        rmq_query_opts_t o{kDefaultShortMax, kDefaultGroupLanes, 0, kDefaultLongLanes};
+        rmq_query_opts_t o{kBaseShortMax, 16, 0, 4};
         return o;
     };
     constexpr int kReps = 5;     // timed passes per configuration (best kept)
@@ -1322,7 +1325,7 @@ RMQ_API int rmq_tune_ex(const float *d_A, int64_t n, const int32_t *d_Q, int64_t
             if (rc == RMQ_ERR_OOM && flags) continue;  // no room for edge tables: plain only
             if (rc != RMQ_OK) return rc;
             float ms = 1e30f;
-            for (int32_t sm : {1, 32, kDefaultShortMax}) {
+            for (int32_t sm : {1, 32, kBaseShortMax}) {
                 if (sm == 1 && !flags) continue;
                 rmq_query_opts_t o = base_opts(bs);
                 o.short_max = sm;

@@ -27,8 +27,13 @@ constexpr int64_t kTwoStageTableBytes = int64_t(64) << 20;  // block tables beyo
 // little in these memory-bound kernels.
 #define RMQ_SGPR_BUDGET __attribute__((amdgpu_num_sgpr(72)))
 constexpr int kLongUnroll = 4;           // float4 loads in flight per lane in the wave scan
-constexpr int kDefaultShortMax = 1024;
-constexpr int kDefaultGroupLanes = 16;
+// Default short-range threshold and lanes per short range: ranges up to 128
+// elements are scanned by 8 lanes, longer ones inside a block read their whole
+// sub-blocks as minima (plain handle at bs=2^15, n=1e8, 2^26 queries per batch,
+// uniform / lognormal 2^14 / lognormal 2^6 lengths: 1.02 + 5.05 + 6.34 ms at
+// 1024 and 16 lanes, 1.00 + 4.92 + 4.45 ms at 128 and 8).
+constexpr int kDefaultShortMax = 128;
+constexpr int kDefaultGroupLanes = 8;
 // Lanes per long query by default: with the sparse table over sub-block
 // minima a long in-block scan is two partial sub-blocks and two entries, so
 // more queries in flight beat more lanes per query (literal bs=2^15,

@@ -232,7 +232,7 @@ def test_query_kernel_names(gpu):
         assert ix.kernels(opts) == ["k_query_edges<8, true>"]
     with R.RMQIndex(A, 1 << 15) as ix:
         assert ix.kernels(opts) == ["k_query_tiles<8, false>", "k_query_tiles<16, true>"]
-        assert ix.kernels() == ["k_query_tiles<16, false>", "k_query_tiles<4, true>"]
+        assert ix.kernels() == ["k_query_tiles<8, false>", "k_query_tiles<4, true>"]  # the defaults
 
 
 def test_autotuner_pick_and_cache(tmp_path, gpu):
