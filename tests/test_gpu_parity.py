@@ -233,6 +233,8 @@ def test_query_kernel_names(gpu):
     with R.RMQIndex(A, 1 << 15) as ix:
         assert ix.kernels(opts) == ["k_query_tiles<8, false>", "k_query_tiles<16, true>"]
         assert ix.kernels() == ["k_query_tiles<8, false>", "k_query_tiles<4, true>"]  # the defaults
+        o = ix.default_opts()
+        assert (o.short_max, o.group_lanes, o.long_wgs, o.long_lanes) == (128, 8, 0, 4)
 
 
 def test_autotuner_pick_and_cache(tmp_path, gpu):
