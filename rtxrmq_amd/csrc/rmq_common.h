@@ -748,18 +748,26 @@ __device__ __forceinline__ void answer_class(const Blocks &k, uint64_t mk, const
     constexpr int U = (G == kWave) ? (LONG ? kLongUnroll : 2) : (LONG ? 2 : U_SHORT);
     constexpr bool PRED = !LONG && G < kWave && PRED_SHORT;
     if constexpr (LONG) {
-        // One lane per query first: a range across blocks whose blocks'
-        // minimum has its argmin inside the range is answered here by two
-        // table reads; only the others go through the G-lane path.
+        // One lane per query first: a range whose enclosing blocks (across
+        // blocks) or enclosing 32-element sub-blocks (inside one block) have
+        // their leftmost minimum inside the range is answered here by two
+        // table reads -- that minimum is then the range's, and no earlier
+        // index of the range can hold it; only the others go through the
+        // G-lane path.
         bool done = false;
         if ((mk >> lane) & 1) {
             const int lB = block_of(m.l, k.div), rB = block_of(m.r, k.div);
+            Pair sup{INFINITY, -1};
             if (lB != rB) {
-                const Pair sup = table_min(k.st, k.nb, lB, rB - lB + 1);
-                if (sup.i >= m.l && sup.i <= m.r) {
-                    write_result<NT>(out_v, out_i, qbase + lane, sup.v, sup.i);
-                    done = true;
-                }
+                sup = table_min(k.st, k.nb, lB, rB - lB + 1);
+            } else if (k.sub) {
+                const int base = lB * k.bs;
+                const int jl = (m.l - base) >> kSubShift, jr = (m.r - base) >> kSubShift;
+                sup = table_min(k.sub, (uint32_t)(k.nb * k.nsub), (int64_t)lB * k.nsub + jl, jr - jl + 1);
+            }
+            if (sup.i >= m.l && sup.i <= m.r) {
+                write_result<NT>(out_v, out_i, qbase + lane, sup.v, sup.i);
+                done = true;
             }
         }
         mk &= ~(uint64_t)__ballot(done);
