@@ -251,7 +251,18 @@ def algorithmic_bytes(Q, lvl: BlockLevel, short_max: int, out_bytes: int = 8, ed
             gathers += 2 * int(slow.sum())
             reads += 2 * int(ls.numel()) + 2 * int((slow & (gr - gl > 1)).sum())
         else:
-            long_b += float((8 + scan_bytes(l[same], r[same], bs) + out_bytes).sum())
+            ls, rs = l[same], r[same]
+            if sub is not None and bs % 32 == 0:
+                # the one-lane test over the enclosing sub-blocks: two entries,
+                # and the scan only when their leftmost minimum lies outside
+                gl, gr = ls // 32, rs // 32
+                sup_i = sub.range_min(gl, gr - gl + 1)[1]
+                hit = (sup_i >= ls) & (sup_i <= rs)
+                reads += 2 * int(ls.numel())
+                long_b += float((8 + 16 + out_bytes) * int(hit.sum()))
+                ls, rs = ls[~hit], rs[~hit]
+                long_b += float(16 * int(ls.numel()))
+            long_b += float((8 + scan_bytes(ls, rs, bs) + out_bytes).sum())
         x = ~same
         l, r, lB, rB = l[x], r[x], lB[x], rB[x]
         if edges and mid_shift is not None:
@@ -722,7 +733,7 @@ def summarise(A, Q_dev, q, steps_out, elapsed, levels, bs_for, opts_for, edges_f
         two = bool(e and kname.startswith(("k_query_edges<", "k_query_edges_mid<")) and kname.endswith(", true>"))
         mid = mid_shift_of(levels[bs_for[lr]].nb) if e and kname.startswith("k_query_edges_mid<") else None
         ab = algorithmic_bytes(Q_dev[lr], levels[bs_for[lr]], opts_for[lr].short_max, edges=e,
-                               sub=levels.get(32) if e else None, A=A, lines=lines, two_stage=two, mid_shift=mid)
+                               sub=levels.get(32), A=A, lines=lines, two_stage=two, mid_shift=mid)
         lb, sb, nl = ab["long"], ab["short"], ab["n_long"]
         lit = literal_bytes(Q_dev[lr], bs_for[lr])
         ms = [s[lr]["total_ms"] for s in steps_out]
@@ -1281,6 +1292,8 @@ def main() -> int:
                                            {lr: fixed_opts for lr in DISTS}, Q_dev, out_v, out_i,
                                            args.fixed_steps, 1, f"bs={args.bs}")
     levels = {args.bs: BlockLevel(A, args.bs)}
+    if args.bs % 32 == 0:  # sub-block minima: the plain handle's in-block superset test
+        levels[32] = BlockLevel(A, 32)
     fixed = summarise(A, Q_dev, q, fixed_steps, fixed_elapsed, levels, {lr: args.bs for lr in DISTS},
                       {lr: fixed_opts for lr in DISTS})
     fixed_digest = answers_digest(torch, {lr: fixed_index for lr in DISTS}, {lr: fixed_opts for lr in DISTS}, Q_dev,
@@ -1346,7 +1359,7 @@ def main() -> int:
         f_opts = {lr: fpicks[lr]["opts"] for lr in DISTS}
         f_steps, f_elapsed = timed_run(R, torch, f_index_for, f_opts, Q_dev, out_v, out_i, args.fixed_steps, 1,
                                        f"bs={args.bs} tuned")
-        if any(p["flags"] for p in fpicks.values()) and 32 not in levels:
+        if 32 not in levels:
             levels[32] = BlockLevel(A, 32)
         fixed_tuned = summarise(A, Q_dev, q, f_steps, f_elapsed, levels, {lr: args.bs for lr in DISTS}, f_opts,
                                 {lr: fpicks[lr]["flags"] for lr in DISTS},
